@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""bench.py -- pixel-geodesics/s of the per-pixel ray tracer on one MI355X.
+
+One step = one render of the 1024x1024 image of a 256x256 HARM torus dump at
+230 GHz (inclination 60 deg, 40 M field of view, FP64 geodesics and
+transfer): every pixel's null geodesic plus its radiative transfer.  Inputs
+are resident in HBM before the timed region; outputs stay in HBM.
+
+Prints one JSON line (rank 0).  Extra fields:
+  roofline      algorithmic FLOP rate of the trace kernel vs the FP64 vector peak
+  grad          image + d/d(ln M_unit, R_high) throughput beside plain images
+  cpu_baseline  the FP64 CPU oracle (OpenMP, all host cores) on a pixel sample
+  parity        GPU vs oracle on that same sample
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--cpu-sample N]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+import warnings
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "pixel-geodesics/s at 1024^2, 230 GHz; image+grad/s vs plain, <=1e-4 flux err"
+FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector (dense) peak
+
+WORKLOAD = dict(workload="harm2d-torus-256x256_img1024x1024_230GHz_i60", dump="256x256x1",
+                image="1024x1024", freq_hz=2.3e11, inclination_deg=60.0, M_unit=1e23,
+                R_high=10.0, R_low=1.0, stepsize=0.03, fov_M=40.0)
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--grad-steps", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=int, default=65536,
+                    help="pixels of the workload traced by the CPU oracle (0 = skip)")
+    ap.add_argument("--size", type=int, default=1024)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(pixels shard trivially; see the sharding design). Use --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from raptor_amd import flops, rt, torus
+
+    W = args.size
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        dump = torus.fishbone_moncrief(256, 256)
+    tr = rt.Tracer(0)
+    tr.set_params(width=W, height=W, M_unit=WORKLOAD["M_unit"], R_high=WORKLOAD["R_high"],
+                  R_low=WORKLOAD["R_low"], stepsize=WORKLOAD["stepsize"],
+                  cam_size=WORKLOAD["fov_M"])
+    tr.load_dump(dump)
+    incl = [WORKLOAD["inclination_deg"]]
+    freq = [WORKLOAD["freq_hz"]]
+    npix = W * W
+    d_img = tr.alloc(npix * 8)
+    d_dimg = tr.alloc(npix * 16)
+    d_cnt = tr.alloc(npix * 8)
+
+    # counters pass (untimed): per-ray steps and transfer samples
+    tr.render_device(incl, freq, d_img, None, d_cnt)
+    tr.synchronize()
+    cnt = tr.d2h(d_cnt, (npix, 2), np.int32)
+    alg_flops = flops.image_flops(cnt[:, 0], cnt[:, 1], 1)
+    alg_flops_grad = flops.image_flops(cnt[:, 0], cnt[:, 1], 1, grad=True)
+
+    for _ in range(args.warmup):
+        tr.render_device(incl, freq, d_img)
+    tr.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tr.render_device(incl, freq, d_img)
+        tr.synchronize()
+        kms.append(tr.last_kernel_ms())
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    ms_per_step = 1e3 * wall / args.steps
+    value = npix * args.steps / wall
+    kern_ms = float(np.mean(kms))
+    achieved = alg_flops / (kern_ms * 1e-3) / 1e12
+
+    # gradient leg: image + its two parameter derivatives
+    tr.render_device(incl, freq, d_img, d_dimg)
+    tr.synchronize()
+    tg0 = time.perf_counter()
+    gms = []
+    for _ in range(args.grad_steps):
+        tr.render_device(incl, freq, d_img, d_dimg)
+        tr.synchronize()
+        gms.append(tr.last_kernel_ms())
+    tg1 = time.perf_counter()
+    grad_rate = args.grad_steps / (tg1 - tg0)
+    img = tr.d2h(d_img, (npix,), np.float64)
+    jy = tr.jansky_factor()
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pixel-geodesics/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Fishbone-Moncrief torus, HARM MKS, a=0.9375)",
+        "config": dict(WORKLOAD, image="%dx%d" % (W, W)),
+        "images_per_s": 1e3 / ms_per_step,
+        "flux_jy": float(img.sum() * jy),
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel": "trace_kernel<1,false,false>",
+            "kernel_ms": kern_ms,
+            "alg_flops_per_launch": alg_flops,
+            "rk2_steps_per_launch": int(cnt[:, 0].sum()),
+            "samples_per_launch": int(cnt[:, 1].sum()),
+        },
+        "grad": {
+            "image_plus_grad_per_s": grad_rate,
+            "image_per_s": 1e3 / ms_per_step,
+            "ratio": grad_rate / (1e3 / ms_per_step),
+            "kernel_ms": float(np.mean(gms)),
+            "achieved_tflops": alg_flops_grad / (np.mean(gms) * 1e-3) / 1e12,
+        },
+    }
+
+    if args.cpu_sample > 0:
+        from oracle import oracle as O
+        n = min(args.cpu_sample, npix)
+        stride = max(1, npix // n)
+        pix = np.arange(0, npix, stride, dtype=np.int32)[:n]
+        P = O.params(dump.grid, width=W, height=W, inclination=incl[0],
+                     M_unit=WORKLOAD["M_unit"], R_high=WORKLOAD["R_high"],
+                     R_low=WORKLOAD["R_low"], cam_size=WORKLOAD["fov_M"],
+                     stepsize=WORKLOAD["stepsize"])
+        cores = host_cores()
+        c0 = time.perf_counter()
+        ora = O.render(P, dump.prims, freq, pixels=pix, nthreads=cores)[:, 0]
+        c1 = time.perf_counter()
+        cpu_rate = n / (c1 - c0)
+        result["cpu_baseline"] = {
+            "value": cpu_rate, "unit": "pixel-geodesics/s", "cores": cores, "kind": "port",
+            "sample": "%d of %d pixels (every %d-th) of the same workload, FP64 oracle, "
+                      "OpenMP dynamic schedule, %.1f s wall" % (n, npix, stride, c1 - c0),
+        }
+        result["speedup_vs_cpu"] = value / cpu_rate
+        g = img[pix]
+        floor = 1e-6 * np.max(np.abs(ora))
+        r = np.abs(g - ora) / np.maximum(np.abs(ora), floor)
+        result["parity"] = {
+            "pixels": int(n), "flux_rel": float(g.sum() / ora.sum() - 1.0),
+            "max_pix_rel": float(r.max()), "n_pix_over_1e-3": int((r > 1e-3).sum()),
+            "floor": "1e-6*max(I)",
+        }
+
+    tr.free(d_img)
+    tr.free(d_dimg)
+    tr.free(d_cnt)
+    tr.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,175 @@
+/*
+ * raptor_rt.h -- C-ABI of the MI355X per-pixel ray tracer.
+ *
+ * This library replaces the pixel loop of RAPTOR 1.0,
+ *     void calculate_image(real **intensityfield, real energy_spectrum[],
+ *                          real frequencies[])          (functions.h:70, core.c:74-152)
+ * and the per-sample model hook behind it,
+ *     int get_fluid_params(real X[4], real *Ne, real *Thetae, real *B, ...)
+ *                                                      (functions.h:24-25, raptor_harm_model.c:190-320)
+ * with one HIP implementation on one GPU.  Everything the reference passes
+ * through globals (parameters.h:58-189, constants.h:32-35) is passed here
+ * explicitly; there is no global state, and one context serves one host
+ * thread at a time.
+ *
+ * Conventions
+ *  - Every function returns RT_OK (0) or a negative RT_E* code; the message
+ *    is available from rt_last_error(ctx) (rt_last_error(NULL) for errors of
+ *    rt_create).  Nothing calls exit().
+ *  - Plain pointers and sizes only.  Pointers named h_* are host memory,
+ *    d_* device memory of the context's device.
+ *  - Images are I_nu in cgs (erg s^-1 cm^-2 Hz^-1 sr^-1) after the nu^3
+ *    scaling of core.c:136, laid out [camera][y*W + x][frequency] like
+ *    intensityfield[y*W + x][f] (core.c:108-113); spectra are sums over pixels
+ *    (core.c:146-150).  JANSKY_FACTOR (utilities.c:32-42) is applied by the
+ *    writer, not here; rt_jansky_factor() returns it.
+ *  - All W*H pixels are traced (the reference silently skips pixel indices
+ *    >= 1e6, core.c:81-103).
+ */
+#ifndef RAPTOR_RT_H
+#define RAPTOR_RT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RT_OK 0
+#define RT_E_INVALID (-1)   /* bad argument or size */
+#define RT_E_HIP (-2)       /* HIP runtime error */
+#define RT_E_STATE (-3)     /* call out of order (e.g. render before load) */
+#define RT_E_NOMEM (-4)     /* host or device allocation failed */
+
+#define RT_MAX_FREQ 16
+#define RT_MAX_CAMERAS 64
+
+typedef struct rt_ctx rt_ctx;
+
+/* HARM grid header (raptor_harm_model.c:90-135 reads it into the globals
+ * startx, dx, a, hslope, R0; parameters.h:80-85).  n3 == 1 is the reference's
+ * 2D case; n3 > 1 is the 3D extension (periodic in X3 = phi). */
+typedef struct {
+    int n1, n2, n3;
+    double startx1, startx2, startx3;
+    double dx1, dx2, dx3;
+    double a, hslope, R0;
+} rt_grid;
+
+/* Run parameters: model.in records (core.c:22-34) and argv (core.c:37-42). */
+typedef struct {
+    double MBH;             /* black-hole mass, g              (model.in MBH) */
+    double M_unit;          /* mass unit, g                    (argv[3])      */
+    double R_high, R_low;   /* R-beta electron model           (argv[5], [6]) */
+    int width, height;      /* pixels                          (IMG_WIDTH/HEIGHT) */
+    double cam_size_x;      /* image size in M                 (CAM_SIZE_X/Y) */
+    double cam_size_y;
+    double stepsize;        /* adaptive step scale             (STEPSIZE)     */
+    int max_steps;          /* step cap, 1e5 in parameters.h:182              */
+} rt_params;
+
+/* Per-ray counters written when requested: RK2 steps taken and samples for
+ * which the transfer was evaluated (inside the grid and past the sigma cut). */
+typedef struct {
+    int32_t steps;
+    int32_t samples;
+} rt_ray_counters;
+
+/* ---- lifetime ---------------------------------------------------------- */
+
+/* Create a context on HIP device `device`; NULL on failure. */
+rt_ctx *rt_create(int device);
+void rt_destroy(rt_ctx *ctx);
+const char *rt_last_error(const rt_ctx *ctx);
+/* Library version string. */
+const char *rt_version(void);
+
+/* ---- set-up (once per run / dump) --------------------------------------- */
+
+/* Replaces the globals set by read_model (core.c:4-72) and set_units
+ * (grmhd.c:17-33).  May be called again between renders. */
+int rt_set_params(rt_ctx *ctx, const rt_params *p);
+
+/* Replaces init_model()'s storage step (raptor_harm_model.c:59-66,
+ * grmhd.c:76-91): copies the 8 primitives KRHO, UU, U1, U2, U3, B1, B2, B3
+ * (raptor_harm_model.h:67-74) to HBM once.  h_prims is float32 in the
+ * reference's index order p[v][i][j][k], i.e. h_prims[((v*n1+i)*n2+j)*n3+k].
+ * Requires n1, n2 >= 2, n3 >= 1, R0 == 0 (the reference's inverse metric
+ * assumes it, metric.c:207-231). */
+int rt_load_prims(rt_ctx *ctx, const rt_grid *grid, const float *h_prims);
+
+/* ---- the pixel loop ----------------------------------------------------- */
+
+/* calculate_image for n_cam inclinations (degrees) and n_freq frequencies
+ * (Hz) in one launch.  h_image: [n_cam][W*H][n_freq] doubles; h_spectrum
+ * (may be NULL): [n_cam][n_freq].  Synchronous. */
+int rt_render(rt_ctx *ctx, const double *inclinations, int n_cam,
+              const double *freqs, int n_freq, double *h_image, double *h_spectrum);
+
+/* Same, plus the image's derivatives with respect to ln(M_unit) and R_high:
+ * h_dimage [n_cam][W*H][n_freq][2] (index 0: d/d ln M_unit, 1: d/d R_high). */
+int rt_render_grad(rt_ctx *ctx, const double *inclinations, int n_cam,
+                   const double *freqs, int n_freq, double *h_image, double *h_dimage,
+                   double *h_spectrum);
+
+/* Device-resident variant: writes d_image (and d_dimage if non-NULL, and
+ * d_counters [n_cam*W*H] if non-NULL) in HBM on the context's stream and
+ * returns without synchronising.  The buffers must hold the sizes above. */
+int rt_render_device(rt_ctx *ctx, const double *inclinations, int n_cam,
+                     const double *freqs, int n_freq, double *d_image, double *d_dimage,
+                     rt_ray_counters *d_counters);
+
+/* Wait for the context's stream. */
+int rt_synchronize(rt_ctx *ctx);
+
+/* Duration in ms of the last render's trace kernel(s), from HIP events on
+ * the context's stream (valid after rt_synchronize). */
+double rt_last_kernel_ms(const rt_ctx *ctx);
+
+/* JANSKY_FACTOR of utilities.c:32-42 for the current parameters. */
+double rt_jansky_factor(const rt_ctx *ctx);
+
+/* ---- launch configuration ------------------------------------------------ */
+
+/* Threads per block and resident blocks per CU of the persistent trace
+ * kernel (0 = library default), and rays fetched per queue claim. */
+typedef struct {
+    int block;
+    int blocks_per_cu;
+    int chunk;
+} rt_launch_config;
+
+int rt_set_launch_config(rt_ctx *ctx, const rt_launch_config *cfg);
+int rt_get_launch_config(const rt_ctx *ctx, rt_launch_config *cfg);
+
+/* ---- device buffers for callers that keep outputs in HBM ---------------- */
+int rt_device_alloc(rt_ctx *ctx, size_t bytes, void **d_ptr);
+int rt_device_free(rt_ctx *ctx, void *d_ptr);
+int rt_memcpy_d2h(rt_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
+
+/* ---- the model hook on the device --------------------------------------- */
+
+/* get_fluid_params (raptor_harm_model.c:190-320) evaluated by the device
+ * sampler at n points X[n][4] (MKS).  out[n][12]: Ne (cm^-3), Thetae, B (G),
+ * Ucon[4], Bcon[4], and the hook's return value (1, or 0 for b^2/rho > 1). */
+int rt_fluid_params(rt_ctx *ctx, const double *h_X, int n, double *h_out);
+
+/* Device evaluation of the path's building blocks at n points, for checking:
+ *   RT_PROBE_METRIC     in X[4]          -> out g_dd[16], g_uu[16]   (metric.c:94-138, 207-231)
+ *   RT_PROBE_ACCEL      in X[4], k[4]    -> out A[4] = -Gamma^m_ab k^a k^b (integrator.c:126-146)
+ *   RT_PROBE_RK2        in y[8]          -> out dl, y'[8] (integrator.c:44-70, 109-122)
+ *   RT_PROBE_CAMERA     in alpha, beta, inclination -> out y[8]     (metric.c:650-727)
+ *   RT_PROBE_EMISSION   in B, cos(pitch), Thetae, nu_p, Ne -> out j, alpha_nu (j_nu.c:130-200)
+ */
+#define RT_PROBE_METRIC 1
+#define RT_PROBE_ACCEL 2
+#define RT_PROBE_RK2 3
+#define RT_PROBE_CAMERA 4
+#define RT_PROBE_EMISSION 5
+int rt_probe(rt_ctx *ctx, int kind, const double *h_in, int n, double *h_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RAPTOR_RT_H */

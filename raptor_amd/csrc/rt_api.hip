@@ -1,0 +1,751 @@
+// rt_api.hip -- trace kernels and the C-ABI of include/raptor_rt.h.
+//
+// The trace kernel is persistent: the grid is sized to the resident
+// capacity of the GPU, and every lane claims rays from one global atomic
+// counter, tracing each ray to completion and claiming the next one as soon
+// as its current ray ends.  Ray lengths differ by orders of magnitude
+// (escaping rays ~400 RK2 steps, rays skimming the photon ring thousands),
+// so a lane-granular queue keeps the CUs busy until the last ray is claimed.
+//
+// A ray id is cam * W*H + (y*W + x); its outputs go to image[id*nf + f].
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/raptor_rt.h"
+#include "rt_physics.hpp"
+
+using namespace rt;
+
+namespace {
+
+struct TraceArgs {
+    const float4 *prims;
+    unsigned int *queue;
+    int total;
+    int chunk;
+    double *image;
+    double *dimage;
+    rt_ray_counters *counters;
+    double nu3[kMaxFreq];
+};
+
+template <int NF, bool GRAD, bool DIM3>
+__global__ __launch_bounds__(64) void trace_kernel(const Params P, const TraceArgs A)
+{
+    Ray<NF, GRAD> R;
+    const int total = A.total;
+    int ray = (int)atomicAdd(A.queue, (unsigned)A.chunk);
+    int left = ray < total ? min(A.chunk, total - ray) : 0;
+    if (left > 0)
+        ray_init<NF, GRAD>(P, ray / P.npix, ray % P.npix, R);
+    while (left > 0) {
+        bool cont = ray_alive<NF, GRAD>(P, R);
+        if (cont)
+            cont = ray_step<NF, GRAD, DIM3>(P, A.prims, R);
+        if (!cont) {
+            const size_t o = (size_t)ray * P.nf;
+            for (int f = 0; f < NF; f++) {
+                if (f < P.nf) {
+                    A.image[o + f] = R.acc.I[f] * A.nu3[f];
+                    if (GRAD) {
+                        A.dimage[(o + f) * 2 + 0] = R.acc.dI[f][0] * A.nu3[f];
+                        A.dimage[(o + f) * 2 + 1] = R.acc.dI[f][1] * A.nu3[f];
+                    }
+                }
+            }
+            if (A.counters) {
+                rt_ray_counters c;
+                c.steps = R.steps;
+                c.samples = R.samples;
+                A.counters[ray] = c;
+            }
+            ray++;
+            left--;
+            if (left == 0) {
+                ray = (int)atomicAdd(A.queue, (unsigned)A.chunk);
+                left = ray < total ? min(A.chunk, total - ray) : 0;
+            }
+            if (left > 0)
+                ray_init<NF, GRAD>(P, ray / P.npix, ray % P.npix, R);
+        }
+    }
+}
+
+__global__ void reset_queue_kernel(unsigned int *q) { *q = 0u; }
+
+// ---- probes: one thread per point ----------------------------------------
+__global__ void probe_metric(const Params P, const double *in, int n, double *out)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    const double *X = in + 4 * t;
+    Point pt = make_point(X[1], X[2], P.hslope);
+    MetricDD g = metric_dd(pt, P.a, P.hslope);
+    double *o = out + 32 * t;
+    for (int i = 0; i < 32; i++)
+        o[i] = 0.;
+    o[0] = g.g00;
+    o[1] = o[4] = g.g01;
+    o[3] = o[12] = g.g03;
+    o[5] = g.g11;
+    o[7] = o[13] = g.g13;
+    o[10] = g.g22;
+    o[15] = g.g33;
+    double r = pt.r, a = P.a;
+    double irho2 = 1. / (r * r + a * a * pt.c * pt.c);
+    double hfac = kPi + (1. - P.hslope) * kPi * pt.c2;
+    double *u = o + 16;
+    u[0] = -1. - 2. * r * irho2;
+    u[1] = u[4] = 2. * irho2;
+    u[5] = irho2 * (r * (r - 2.) + a * a) / (r * r);
+    u[7] = u[13] = a * irho2 / r;
+    u[10] = irho2 / (hfac * hfac);
+    u[15] = irho2 / (pt.s * pt.s);
+}
+
+__global__ void probe_accel(const Params P, const double *in, int n, double *out)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    const double *X = in + 8 * t;
+    Point pt = make_point(X[1], X[2], P.hslope);
+    geodesic_accel(pt, P.a, P.hslope, X + 4, out + 4 * t);
+}
+
+__global__ void probe_rk2(const Params P, const double *in, int n, double *out)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    double y[8];
+    for (int i = 0; i < 8; i++)
+        y[i] = in[8 * t + i];
+    Point pt = make_point(y[1], y[2], P.hslope);
+    double dl = stepsize(y, P.stepsize);
+    rk2_step(y, pt, dl, P.a, P.hslope);
+    out[9 * t] = dl;
+    for (int i = 0; i < 8; i++)
+        out[9 * t + 1 + i] = y[i];
+}
+
+__global__ void probe_camera(const Params P, const double *in, int n, double *out)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    int cam = (int)in[3 * t + 2];
+    init_photon(P.cam[cam], in[3 * t], in[3 * t + 1], out + 8 * t);
+}
+
+__global__ void probe_emission(const double *in, int n, double *out)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    const double *q = in + 5 * t;
+    double j, an, X;
+    emission(q[0], q[1], q[2], q[3], q[4], j, an, X);
+    out[2 * t] = j;
+    out[2 * t + 1] = an;
+}
+
+template <bool DIM3>
+__global__ void probe_fluid(const Params P, const float4 *prims, const double *in, int n,
+                            double *out)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    double y[8];
+    for (int i = 0; i < 4; i++) {
+        y[i] = in[4 * t + i];
+        y[4 + i] = 0.;
+    }
+    y[4] = 1.;
+    Point pt = make_point(y[1], y[2], P.hslope);
+    Fluid F;
+    double u[4], b[4];
+    bool ok = fluid_sample<DIM3>(P, prims, pt, y, F, u, b);
+    double *o = out + 12 * t;
+    o[0] = ok ? F.ne : kSmall;
+    o[1] = ok ? F.thetae : kSmall;
+    o[2] = ok ? F.B : kSmall;
+    for (int i = 0; i < 4; i++) {
+        o[3 + i] = u[i];
+        o[7 + i] = b[i];
+    }
+    o[11] = ok ? 1. : 0.;
+}
+
+// cell-interleaved staging: 8 primitives of one cell in two float4
+__global__ void stage_prims(const float *soa, size_t ncell, float4 *aos)
+{
+    size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell)
+        return;
+    aos[2 * c] = make_float4(soa[c], soa[ncell + c], soa[2 * ncell + c], soa[3 * ncell + c]);
+    aos[2 * c + 1] = make_float4(soa[4 * ncell + c], soa[5 * ncell + c], soa[6 * ncell + c],
+                                 soa[7 * ncell + c]);
+}
+
+}  // namespace
+
+struct rt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int num_cu = 0;
+    bool have_params = false, have_grid = false;
+    rt_params params{};
+    rt_grid grid{};
+    float4 *d_prims = nullptr;
+    unsigned int *d_queue = nullptr;
+    double *d_image = nullptr, *d_dimage = nullptr;
+    size_t image_cap = 0, dimage_cap = 0;
+    double last_ms = 0.;
+    rt_launch_config cfg{64, 0, 1};
+    char err[512] = {0};
+};
+
+static char g_create_err[512];
+
+static int fail(rt_ctx *c, int code, const char *fmt, ...)
+{
+    char *buf = c ? c->err : g_create_err;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, 512, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(ctx, RT_E_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                           \
+    } while (0)
+
+static int select_device(rt_ctx *c)
+{
+    HIPCHK(c, hipSetDevice(c->device));
+    return RT_OK;
+}
+
+// Fills the launch parameters from the context; validates sizes.
+static int make_params(rt_ctx *c, const double *incl, int ncam, const double *freqs, int nf,
+                       Params &P, double nu3[kMaxFreq])
+{
+    if (!c->have_params)
+        return fail(c, RT_E_STATE, "rt_set_params has not been called");
+    if (!c->have_grid)
+        return fail(c, RT_E_STATE, "rt_load_prims has not been called");
+    if (ncam < 1 || ncam > kMaxCam)
+        return fail(c, RT_E_INVALID, "n_cam must be in [1, %d], got %d", kMaxCam, ncam);
+    if (nf < 1 || nf > kMaxFreq)
+        return fail(c, RT_E_INVALID, "n_freq must be in [1, %d], got %d", kMaxFreq, nf);
+    if (!incl || !freqs)
+        return fail(c, RT_E_INVALID, "inclinations and freqs must not be NULL");
+    const rt_params &p = c->params;
+    const rt_grid &g = c->grid;
+    long long total = (long long)ncam * p.width * p.height;
+    if (total > 0x7fffff00LL)
+        return fail(c, RT_E_INVALID, "too many rays in one launch (%lld)", total);
+    const char *msg = fill_params(p, g, incl, ncam, freqs, nf, P, nu3);
+    if (msg)
+        return fail(c, RT_E_INVALID, "%s", msg);
+    return RT_OK;
+}
+
+template <int NF, bool GRAD, bool DIM3>
+static hipError_t launch_trace(rt_ctx *c, const Params &P, const TraceArgs &A)
+{
+    auto kern = trace_kernel<NF, GRAD, DIM3>;
+    int block = 64;
+    int per_cu = c->cfg.blocks_per_cu;
+    if (per_cu <= 0) {
+        int occ = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, block, 0);
+        if (e != hipSuccess)
+            return e;
+        per_cu = std::max(occ, 1);
+    }
+    long long grid = (long long)per_cu * c->num_cu;
+    long long need = ((long long)A.total + A.chunk - 1) / A.chunk;
+    grid = std::max(1LL, std::min(grid, need));
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), 0, c->stream, P, A);
+    return hipGetLastError();
+}
+
+template <bool GRAD, bool DIM3>
+static hipError_t dispatch_nf(rt_ctx *c, const Params &P, const TraceArgs &A)
+{
+    if (P.nf == 1)
+        return launch_trace<1, GRAD, DIM3>(c, P, A);
+    if (P.nf <= 4)
+        return launch_trace<4, GRAD, DIM3>(c, P, A);
+    if (P.nf <= 8)
+        return launch_trace<8, GRAD, DIM3>(c, P, A);
+    return launch_trace<16, GRAD, DIM3>(c, P, A);
+}
+
+static int ensure_buffer(rt_ctx *c, double **buf, size_t *cap, size_t bytes)
+{
+    if (*cap >= bytes)
+        return RT_OK;
+    if (*buf)
+        HIPCHK(c, hipFree(*buf));
+    *buf = nullptr;
+    *cap = 0;
+    HIPCHK(c, hipMalloc((void **)buf, bytes));
+    *cap = bytes;
+    return RT_OK;
+}
+
+static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *freqs, int nf,
+                       double *d_image, double *d_dimage, rt_ray_counters *d_counters, bool grad)
+{
+    if (!c)
+        return RT_E_INVALID;
+    Params P;
+    TraceArgs A;
+    int rc = make_params(c, incl, ncam, freqs, nf, P, A.nu3);
+    if (rc)
+        return rc;
+    if (!d_image || (grad && !d_dimage))
+        return fail(c, RT_E_INVALID, "output buffers must not be NULL");
+    if ((rc = select_device(c)))
+        return rc;
+    A.prims = c->d_prims;
+    A.queue = c->d_queue;
+    A.total = ncam * P.npix;
+    A.chunk = std::max(1, c->cfg.chunk);
+    A.image = d_image;
+    A.dimage = grad ? d_dimage : nullptr;
+    A.counters = d_counters;
+    hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    hipError_t e;
+    bool d3 = c->grid.n3 > 1;
+    if (grad)
+        e = d3 ? dispatch_nf<true, true>(c, P, A) : dispatch_nf<true, false>(c, P, A);
+    else
+        e = d3 ? dispatch_nf<false, true>(c, P, A) : dispatch_nf<false, false>(c, P, A);
+    HIPCHK(c, e);
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    return RT_OK;
+}
+
+static int render_host(rt_ctx *c, const double *incl, int ncam, const double *freqs, int nf,
+                       double *h_image, double *h_dimage, double *h_spectrum, bool grad)
+{
+    if (!c)
+        return RT_E_INVALID;
+    if (!h_image || (grad && !h_dimage))
+        return fail(c, RT_E_INVALID, "output buffers must not be NULL");
+    if (!c->have_params || !c->have_grid)
+        return fail(c, RT_E_STATE, "rt_set_params and rt_load_prims must be called first");
+    if (ncam < 1 || ncam > kMaxCam || nf < 1 || nf > kMaxFreq)
+        return fail(c, RT_E_INVALID, "n_cam or n_freq out of range");
+    int rc = select_device(c);
+    if (rc)
+        return rc;
+    size_t n = (size_t)ncam * c->params.width * c->params.height * nf;
+    if ((rc = ensure_buffer(c, &c->d_image, &c->image_cap, n * sizeof(double))))
+        return rc;
+    if (grad && (rc = ensure_buffer(c, &c->d_dimage, &c->dimage_cap, 2 * n * sizeof(double))))
+        return rc;
+    rc = render_impl(c, incl, ncam, freqs, nf, c->d_image, grad ? c->d_dimage : nullptr,
+                     nullptr, grad);
+    if (rc)
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(h_image, c->d_image, n * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    if (grad)
+        HIPCHK(c, hipMemcpyAsync(h_dimage, c->d_dimage, 2 * n * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    if (h_spectrum) {
+        // core.c:146-150: sequential sum over pixels
+        size_t npix = (size_t)c->params.width * c->params.height;
+        for (int k = 0; k < ncam; k++)
+            for (int f = 0; f < nf; f++) {
+                double s = 0.;
+                for (size_t i = 0; i < npix; i++)
+                    s += h_image[((size_t)k * npix + i) * nf + f];
+                h_spectrum[k * nf + f] = s;
+            }
+    }
+    return RT_OK;
+}
+
+extern "C" {
+
+const char *rt_version(void) { return "raptor_rt 0.1 (gfx950)"; }
+
+rt_ctx *rt_create(int device)
+{
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0) {
+        fail(nullptr, RT_E_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+        return nullptr;
+    }
+    if (device < 0 || device >= ndev) {
+        fail(nullptr, RT_E_INVALID, "device %d out of range [0, %d)", device, ndev);
+        return nullptr;
+    }
+    rt_ctx *c = new (std::nothrow) rt_ctx();
+    if (!c) {
+        fail(nullptr, RT_E_NOMEM, "out of host memory");
+        return nullptr;
+    }
+    c->device = device;
+    hipDeviceProp_t prop;
+    if ((e = hipSetDevice(device)) != hipSuccess ||
+        (e = hipGetDeviceProperties(&prop, device)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess ||
+        (e = hipMalloc((void **)&c->d_queue, sizeof(unsigned int))) != hipSuccess) {
+        fail(nullptr, RT_E_HIP, "HIP set-up failed: %s", hipGetErrorString(e));
+        rt_destroy(c);
+        return nullptr;
+    }
+    c->num_cu = prop.multiProcessorCount;
+    return c;
+}
+
+void rt_destroy(rt_ctx *c)
+{
+    if (!c)
+        return;
+    (void)hipSetDevice(c->device);
+    if (c->stream)
+        (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->d_prims);
+    (void)hipFree(c->d_queue);
+    (void)hipFree(c->d_image);
+    (void)hipFree(c->d_dimage);
+    if (c->ev0)
+        (void)hipEventDestroy(c->ev0);
+    if (c->ev1)
+        (void)hipEventDestroy(c->ev1);
+    if (c->stream)
+        (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *rt_last_error(const rt_ctx *c) { return c ? c->err : g_create_err; }
+
+int rt_set_params(rt_ctx *c, const rt_params *p)
+{
+    if (!c)
+        return RT_E_INVALID;
+    if (!p)
+        return fail(c, RT_E_INVALID, "params must not be NULL");
+    if (p->width < 1 || p->height < 1 || (long long)p->width * p->height > 0x7fffff00LL)
+        return fail(c, RT_E_INVALID, "bad image size %d x %d", p->width, p->height);
+    if (!(p->MBH > 0.) || !(p->M_unit > 0.) || !(p->stepsize > 0.) || p->max_steps < 1 ||
+        !(p->cam_size_x > 0.) || !(p->cam_size_y > 0.))
+        return fail(c, RT_E_INVALID, "MBH, M_unit, stepsize, max_steps and camera size must be positive");
+    c->params = *p;
+    c->have_params = true;
+    return RT_OK;
+}
+
+int rt_load_prims(rt_ctx *c, const rt_grid *g, const float *h_prims)
+{
+    if (!c)
+        return RT_E_INVALID;
+    if (!g || !h_prims)
+        return fail(c, RT_E_INVALID, "grid and prims must not be NULL");
+    if (g->n1 < 2 || g->n2 < 2 || g->n3 < 1)
+        return fail(c, RT_E_INVALID, "grid must be at least 2 x 2 x 1, got %d x %d x %d", g->n1,
+                    g->n2, g->n3);
+    if (g->R0 != 0.)
+        return fail(c, RT_E_INVALID, "R0 != 0 is not supported (the MKS inverse metric of the "
+                                     "reference assumes R0 = 0)");
+    if (!(std::fabs(g->a) < 1.) || !(g->dx1 > 0.) || !(g->dx2 > 0.) ||
+        (g->n3 > 1 && !(g->dx3 > 0.)))
+        return fail(c, RT_E_INVALID, "need |a| < 1 and positive cell sizes");
+    int rc = select_device(c);
+    if (rc)
+        return rc;
+    size_t ncell = (size_t)g->n1 * g->n2 * g->n3;
+    if (c->d_prims) {
+        HIPCHK(c, hipFree(c->d_prims));
+        c->d_prims = nullptr;
+    }
+    c->have_grid = false;
+    float *d_soa = nullptr;
+    HIPCHK(c, hipMalloc((void **)&c->d_prims, ncell * 2 * sizeof(float4)));
+    HIPCHK(c, hipMalloc((void **)&d_soa, ncell * 8 * sizeof(float)));
+    hipError_t e = hipMemcpyAsync(d_soa, h_prims, ncell * 8 * sizeof(float),
+                                  hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        unsigned blocks = (unsigned)((ncell + 255) / 256);
+        hipLaunchKernelGGL(stage_prims, dim3(blocks), dim3(256), 0, c->stream, d_soa, ncell,
+                           c->d_prims);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_soa);
+    if (e != hipSuccess)
+        return fail(c, RT_E_HIP, "primitive upload failed: %s", hipGetErrorString(e));
+    c->grid = *g;
+    c->have_grid = true;
+    return RT_OK;
+}
+
+int rt_render(rt_ctx *c, const double *incl, int ncam, const double *freqs, int nf,
+              double *h_image, double *h_spectrum)
+{
+    return render_host(c, incl, ncam, freqs, nf, h_image, nullptr, h_spectrum, false);
+}
+
+int rt_render_grad(rt_ctx *c, const double *incl, int ncam, const double *freqs, int nf,
+                   double *h_image, double *h_dimage, double *h_spectrum)
+{
+    return render_host(c, incl, ncam, freqs, nf, h_image, h_dimage, h_spectrum, true);
+}
+
+int rt_render_device(rt_ctx *c, const double *incl, int ncam, const double *freqs, int nf,
+                     double *d_image, double *d_dimage, rt_ray_counters *d_counters)
+{
+    if (!c)
+        return RT_E_INVALID;
+    return render_impl(c, incl, ncam, freqs, nf, d_image, d_dimage, d_counters,
+                       d_dimage != nullptr);
+}
+
+int rt_synchronize(rt_ctx *c)
+{
+    if (!c)
+        return RT_E_INVALID;
+    int rc = select_device(c);
+    if (rc)
+        return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess)
+        c->last_ms = ms;
+    return RT_OK;
+}
+
+double rt_last_kernel_ms(const rt_ctx *c) { return c ? c->last_ms : 0.; }
+
+double rt_jansky_factor(const rt_ctx *c)
+{
+    if (!c || !c->have_params)
+        return 0.;
+    const rt_params &p = c->params;
+    double Rg = kGGrav * p.MBH / kC / kC;
+    double dx = p.cam_size_x * Rg / kSourceDist;
+    double dy = p.cam_size_y * Rg / kSourceDist;
+    return 1.e23 * (dx / (double)p.width) * (dy / (double)p.height);
+}
+
+int rt_set_launch_config(rt_ctx *c, const rt_launch_config *cfg)
+{
+    if (!c)
+        return RT_E_INVALID;
+    if (!cfg || cfg->blocks_per_cu < 0 || cfg->chunk < 0 || cfg->blocks_per_cu > 64 ||
+        cfg->chunk > 4096)
+        return fail(c, RT_E_INVALID, "bad launch config");
+    c->cfg.block = 64;
+    c->cfg.blocks_per_cu = cfg->blocks_per_cu;
+    c->cfg.chunk = cfg->chunk == 0 ? 1 : cfg->chunk;
+    return RT_OK;
+}
+
+int rt_get_launch_config(const rt_ctx *c, rt_launch_config *cfg)
+{
+    if (!c || !cfg)
+        return RT_E_INVALID;
+    *cfg = c->cfg;
+    return RT_OK;
+}
+
+int rt_device_alloc(rt_ctx *c, size_t bytes, void **d_ptr)
+{
+    if (!c || !d_ptr)
+        return RT_E_INVALID;
+    int rc = select_device(c);
+    if (rc)
+        return rc;
+    HIPCHK(c, hipMalloc(d_ptr, bytes));
+    return RT_OK;
+}
+
+int rt_device_free(rt_ctx *c, void *d_ptr)
+{
+    if (!c)
+        return RT_E_INVALID;
+    int rc = select_device(c);
+    if (rc)
+        return rc;
+    HIPCHK(c, hipFree(d_ptr));
+    return RT_OK;
+}
+
+int rt_memcpy_d2h(rt_ctx *c, void *h_dst, const void *d_src, size_t bytes)
+{
+    if (!c || (!h_dst && bytes) || (!d_src && bytes))
+        return RT_E_INVALID;
+    int rc = select_device(c);
+    if (rc)
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
+int rt_fluid_params(rt_ctx *c, const double *h_X, int n, double *h_out)
+{
+    if (!c)
+        return RT_E_INVALID;
+    if (n < 0 || (n && (!h_X || !h_out)))
+        return fail(c, RT_E_INVALID, "bad point list");
+    if (n == 0)
+        return RT_OK;
+    const double one = 2.3e11, incl = 60.;
+    Params P;
+    double nu3[kMaxFreq];
+    int rc = make_params(c, &incl, 1, &one, 1, P, nu3);
+    if (rc)
+        return rc;
+    if ((rc = select_device(c)))
+        return rc;
+    double *d_in = nullptr, *d_out = nullptr;
+    HIPCHK(c, hipMalloc((void **)&d_in, (size_t)n * 4 * sizeof(double)));
+    hipError_t e = hipMalloc((void **)&d_out, (size_t)n * 12 * sizeof(double));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_in, h_X, (size_t)n * 4 * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream);
+    if (e == hipSuccess) {
+        unsigned blocks = (unsigned)((n + 63) / 64);
+        if (c->grid.n3 > 1)
+            hipLaunchKernelGGL(probe_fluid<true>, dim3(blocks), dim3(64), 0, c->stream, P,
+                               c->d_prims, d_in, n, d_out);
+        else
+            hipLaunchKernelGGL(probe_fluid<false>, dim3(blocks), dim3(64), 0, c->stream, P,
+                               c->d_prims, d_in, n, d_out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h_out, d_out, (size_t)n * 12 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_in);
+    (void)hipFree(d_out);
+    if (e != hipSuccess)
+        return fail(c, RT_E_HIP, "rt_fluid_params: %s", hipGetErrorString(e));
+    return RT_OK;
+}
+
+int rt_probe(rt_ctx *c, int kind, const double *h_in, int n, double *h_out)
+{
+    if (!c)
+        return RT_E_INVALID;
+    if (n < 0 || (n && (!h_in || !h_out)))
+        return fail(c, RT_E_INVALID, "bad point list");
+    int nin, nout;
+    switch (kind) {
+    case RT_PROBE_METRIC: nin = 4; nout = 32; break;
+    case RT_PROBE_ACCEL: nin = 8; nout = 4; break;
+    case RT_PROBE_RK2: nin = 8; nout = 9; break;
+    case RT_PROBE_CAMERA: nin = 3; nout = 8; break;
+    case RT_PROBE_EMISSION: nin = 5; nout = 2; break;
+    default: return fail(c, RT_E_INVALID, "unknown probe kind %d", kind);
+    }
+    if (n == 0)
+        return RT_OK;
+    Params P;
+    memset(&P, 0, sizeof P);
+    std::vector<double> in(h_in, h_in + (size_t)nin * n);
+    if (kind != RT_PROBE_EMISSION) {
+        if (!c->have_grid || !c->have_params)
+            return fail(c, RT_E_STATE, "rt_set_params and rt_load_prims must be called first");
+        std::vector<double> incl;
+        if (kind == RT_PROBE_CAMERA) {
+            // each point names its inclination; one camera per distinct value
+            for (int i = 0; i < n; i++) {
+                double v = in[3 * i + 2];
+                auto it = std::find(incl.begin(), incl.end(), v);
+                if (it == incl.end()) {
+                    if ((int)incl.size() == kMaxCam)
+                        return fail(c, RT_E_INVALID, "too many distinct inclinations");
+                    incl.push_back(v);
+                    it = incl.end() - 1;
+                }
+                in[3 * i + 2] = (double)(it - incl.begin());
+            }
+        } else {
+            incl.push_back(60.);
+        }
+        const double nu = 2.3e11;
+        double nu3[kMaxFreq];
+        int rc = make_params(c, incl.data(), (int)incl.size(), &nu, 1, P, nu3);
+        if (rc)
+            return rc;
+    }
+    int rc = select_device(c);
+    if (rc)
+        return rc;
+    double *d_in = nullptr, *d_out = nullptr;
+    HIPCHK(c, hipMalloc((void **)&d_in, in.size() * sizeof(double)));
+    hipError_t e = hipMalloc((void **)&d_out, (size_t)n * nout * sizeof(double));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_in, in.data(), in.size() * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream);
+    if (e == hipSuccess) {
+        dim3 grid((unsigned)((n + 63) / 64)), block(64);
+        switch (kind) {
+        case RT_PROBE_METRIC:
+            hipLaunchKernelGGL(probe_metric, grid, block, 0, c->stream, P, d_in, n, d_out);
+            break;
+        case RT_PROBE_ACCEL:
+            hipLaunchKernelGGL(probe_accel, grid, block, 0, c->stream, P, d_in, n, d_out);
+            break;
+        case RT_PROBE_RK2:
+            hipLaunchKernelGGL(probe_rk2, grid, block, 0, c->stream, P, d_in, n, d_out);
+            break;
+        case RT_PROBE_CAMERA:
+            hipLaunchKernelGGL(probe_camera, grid, block, 0, c->stream, P, d_in, n, d_out);
+            break;
+        default:
+            hipLaunchKernelGGL(probe_emission, grid, block, 0, c->stream, d_in, n, d_out);
+            break;
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h_out, d_out, (size_t)n * nout * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_in);
+    (void)hipFree(d_out);
+    if (e != hipSuccess)
+        return fail(c, RT_E_HIP, "rt_probe: %s", hipGetErrorString(e));
+    return RT_OK;
+}
+
+}  // extern "C"

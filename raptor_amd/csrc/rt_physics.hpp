@@ -1,0 +1,717 @@
+// rt_physics.hpp -- per-ray physics of the MI355X tracer (device code).
+//
+// One ray lives in one lane's registers: its state y = (X^mu, k^mu) in FP64,
+// the trigonometry of its current point (shared by the metric of the
+// transfer sample and the first RK2 stage of the next step), and per
+// frequency the optical depth and intensity accumulators.
+//
+// The functions restate the reference's FP64 path (parameters.h:13):
+//   camera            metric.c:650-727, GRmath.c:135-184, newtonraphson.c:76-97
+//   step size         integrator.c:109-122
+//   RK2 midpoint      integrator.c:44-70
+//   geodesic accel.   integrator.c:126-146 with metric.c:538-638 (MKS Gamma)
+//   metric            metric.c:94-138, 207-231
+//   fluid sample      raptor_harm_model.c:190-320, grmhd.c:35-71
+//   transfer          radiative_transfer.c:17-119
+//   emissivity        j_nu.c:15, 130-153, 188-200
+// The connection is contracted with k directly from its 36 non-zero unique
+// entries (Gamma is symmetric in its lower indices) instead of filling a
+// 4x4x4 array, and sin/cos of theta and of 2 pi X2 and exp(X1) are computed
+// once per point.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <cmath>
+
+namespace rt {
+
+// constants.h:16-28, j_nu.c:15, parameters.h:144,170-185
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kElectronCharge = 4.8032068e-10;
+constexpr double kElectronMass = 9.1093826e-28;
+constexpr double kProtonMass = 1.67262171e-24;
+constexpr double kBoltzmann = 1.3806505e-16;
+constexpr double kC = 2.99792458e10;
+constexpr double kPlanck = 6.6260693e-27;
+constexpr double kGGrav = 6.6742e-8;
+constexpr double kE2C = 7.695589048e-30;
+constexpr double kE2CH = 1.161410e-03;
+constexpr double kSourceDist = 2.6228263e22;
+constexpr double kRcam = 1.e4;
+constexpr double kCutoffOuter = kRcam * 1.01;
+constexpr double kHorizonMarg = 1e-1;
+constexpr int kMaxOrder = 100;
+constexpr double kSmall = 1.e-20;
+constexpr int kMaxFreq = 16;
+constexpr int kMaxCam = 64;
+
+// Per-inclination camera constants (the parts of initialize_photon that do
+// not depend on the pixel), computed on the host.
+struct Camera {
+    double sin_i;            // sqrt(1 - mu0^2)
+    double mu0sq;            // mu0^2
+    double cot2;             // cos^2/sin^2 of the camera theta
+    double cosc2;            // cos^2 of the camera theta
+    double g00, g03, g33, g22, g11;   // BL (log-radius) metric at the camera
+    double t01, t31;         // BL -> KS transform of k^t, k^phi
+    double X1, X2, X3;       // camera position (MKS); X0 is never used
+    double thp_inv;          // 1 / theta'(X2) at the camera
+};
+
+// Everything a trace launch reads; passed by value as the kernel argument.
+struct Params {
+    // grid (raptor_harm_model.c:120-135)
+    int n1, n2, n3;
+    double startx1, startx2, startx3;
+    double dx1, dx2, dx3;
+    double stopx1, stopx2;
+    double a, hslope;
+    // camera and integrator
+    int width, height, npix;
+    double cam_x, cam_y;
+    double stepsize;
+    int max_steps;
+    double cutoff_inner;
+    // units (grmhd.c:17-33) and the electron model
+    double Ne_unit, B_unit, Rg;
+    double R_high, R_low;
+    int nf;
+    double freq[kMaxFreq];
+    double dl_scale[kMaxFreq];   // Rg / nu_f
+    int ncam;
+    Camera cam[kMaxCam];
+};
+
+// Trigonometry of one point.
+struct Point {
+    double r;      // exp(X1) (R0 = 0)
+    double s, c;   // sin, cos of theta(X2)
+    double s2, c2; // sin, cos of 2 pi X2
+};
+
+__host__ __device__ inline Point make_point(double X1, double X2, double h)
+{
+    Point p;
+    p.r = exp(X1);
+    sincos(2. * kPi * X2, &p.s2, &p.c2);
+    double th = kPi * X2 + 0.5 * (1. - h) * p.s2;
+    sincos(th, &p.s, &p.c);
+    return p;
+}
+
+// Covariant metric (metric.c:94-138), 10 unique entries:
+// g00 g01 g03 g11 g13 g22 g33 (g02 = g12 = g23 = 0).
+struct MetricDD {
+    double g00, g01, g03, g11, g13, g22, g33;
+};
+
+__host__ __device__ inline MetricDD metric_dd(const Point &p, double a, double h)
+{
+    MetricDD g;
+    double r = p.r;
+    double sin2 = p.s * p.s;
+    double rho2 = r * r + a * a * p.c * p.c;
+    double hfac = kPi + (1. - h) * kPi * p.c2;
+    double z = 2. * r / rho2;
+    g.g00 = -1. + z;
+    g.g01 = z * r;
+    g.g03 = -2. * a * r * sin2 / rho2;
+    g.g11 = (1. + z) * r * r;
+    g.g13 = -a * sin2 * (1. + z) * r;
+    g.g22 = rho2 * hfac * hfac;
+    g.g33 = sin2 * (rho2 + a * a * sin2 * (1. + z));
+    return g;
+}
+
+__host__ __device__ inline void lower(const MetricDD &g, const double v[4], double o[4])
+{
+    o[0] = g.g00 * v[0] + g.g01 * v[1] + g.g03 * v[3];
+    o[1] = g.g01 * v[0] + g.g11 * v[1] + g.g13 * v[3];
+    o[2] = g.g22 * v[2];
+    o[3] = g.g03 * v[0] + g.g13 * v[1] + g.g33 * v[3];
+}
+
+// Geodesic acceleration A^m = -Gamma^m_ab k^a k^b with the closed-form MKS
+// connection of metric.c:538-638.
+__host__ __device__ inline void geodesic_accel(const Point &p, double a, double h,
+                                               const double k[4], double A[4])
+{
+    const double r = p.r, r2 = r * r;
+    const double rp = p.r, rp2 = rp * rp;
+    const double thp = kPi * (1. + (1. - h) * p.c2);
+    const double thp2 = thp * thp;
+    const double thpp = -2. * kPi * kPi * (1. - h) * p.s2;
+    const double c = p.c, s = p.s;
+    const double cos2 = c * c, sin2 = s * s;
+    const double s2th = 2. * s * c;
+    const double cot = c / s;
+    const double a2 = a * a;
+    const double Sig = r2 + a2 * cos2;
+    const double Del = r2 - 2. * r + a2;
+    const double Acap = Sig * Del + 2. * r * (r2 + a2);
+    const double iS = 1. / Sig;
+    const double iS2 = iS * iS, iS3 = iS2 * iS;
+    const double B = (2. * r2 - Sig) * iS3;
+    const double C = r * iS - a2 * B * sin2;
+    const double irp = 1. / rp;
+    const double ithp = 1. / thp;
+
+    // products k^a k^b, off-diagonal ones doubled
+    const double k00 = k[0] * k[0], k11 = k[1] * k[1], k22 = k[2] * k[2], k33 = k[3] * k[3];
+    const double k01 = 2. * k[0] * k[1], k02 = 2. * k[0] * k[2], k03 = 2. * k[0] * k[3];
+    const double k12 = 2. * k[1] * k[2], k13 = 2. * k[1] * k[3], k23 = 2. * k[2] * k[3];
+
+    // Gamma^t
+    const double G000 = 2. * r * B;
+    const double G001 = (Sig + 2. * r) * B * rp;
+    const double G002 = -a2 * r * s2th * iS2 * thp;
+    const double G003 = -2. * a * r * B * sin2;
+    const double G011 = 2. * (Sig + r) * B * rp2;
+    const double G012 = G002 * rp;
+    const double G013 = -a * sin2 * G001;
+    const double G022 = -2. * r2 * iS * thp2;
+    const double G023 = -a * sin2 * G002;
+    const double G033 = -2. * r * C * sin2;
+    A[0] = -(G000 * k00 + G001 * k01 + G002 * k02 + G003 * k03 + G011 * k11 + G012 * k12 +
+             G013 * k13 + G022 * k22 + G023 * k23 + G033 * k33);
+    // Gamma^r (Gamma^1_02 = Gamma^1_23 = 0)
+    const double G100 = Del * B * irp;
+    const double G101 = (Del - Sig) * B;
+    const double G103 = -a * Del * B * sin2 * irp;
+    const double G111 = 1. - (2. * Sig - Del) * B * rp;
+    const double G112 = -a2 * s * c * iS * thp;
+    const double G113 = a * (r * iS + (Sig - Del) * B) * sin2;
+    const double G122 = -r * Del * iS * irp * thp2;
+    const double G133 = -Del * C * sin2 * irp;
+    A[1] = -(G100 * k00 + G101 * k01 + G103 * k03 + G111 * k11 + G112 * k12 + G113 * k13 +
+             G122 * k22 + G133 * k33);
+    // Gamma^theta (Gamma^2_02 = Gamma^2_23 = 0)
+    const double G200 = iS * G002 / thp2;
+    const double G201 = G200 * rp;
+    const double G203 = a * r * (r2 + a2) * s2th * iS3 * ithp;
+    const double G211 = G200 * rp2;
+    const double G212 = r * iS * rp;
+    const double G213 = a * s * c * (Acap + Sig * (Sig - Del)) * iS3 * rp * ithp;
+    const double G222 = thpp * ithp + G112;
+    const double G233 = -s * c * (Del * Sig * Sig + 2. * r * (r2 + a2) * (r2 + a2)) * iS3 * ithp;
+    A[2] = -(G200 * k00 + G201 * k01 + G203 * k03 + G211 * k11 + G212 * k12 + G213 * k13 +
+             G222 * k22 + G233 * k33);
+    // Gamma^phi
+    const double G300 = a * B;
+    const double G301 = G300 * rp;
+    const double G302 = -2. * a * r * cot * iS2 * thp;
+    const double G303 = -a2 * B * sin2;
+    const double G311 = G300 * rp2;
+    const double G312 = -a * (Sig + 2. * r) * cot * iS2 * rp * thp;
+    const double G313 = C * rp;
+    const double G322 = -a * r * iS * thp2;
+    const double G323 = (cot + a2 * r * s2th * iS2) * thp;
+    const double G333 = -a * C * sin2;
+    A[3] = -(G300 * k00 + G301 * k01 + G302 * k02 + G303 * k03 + G311 * k11 + G312 * k12 +
+             G313 * k13 + G322 * k22 + G323 * k23 + G333 * k33);
+}
+
+// integrator.c:109-122 (kept literally, including the 1e-40 guards)
+__host__ __device__ inline double stepsize(const double y[8], double eps)
+{
+    const double S2 = kSmall * kSmall;
+    double dlx1 = eps / (fabs(y[5]) + S2);
+    double dlx2 = eps * fmin(y[2], 1. - y[2]) / (fabs(y[6]) + S2);
+    double dlx3 = eps / (fabs(y[7]) + S2);
+    double i1 = 1. / (fabs(dlx1) + S2);
+    double i2 = 1. / (fabs(dlx2) + S2);
+    double i3 = 1. / (fabs(dlx3) + S2);
+    return -fmax(1. / (i1 + i2 + i3), 1e-12);
+}
+
+// One RK2 midpoint step (integrator.c:44-70).  `pt` holds the trigonometry of
+// y's point on entry and of the new point on exit.
+__host__ __device__ inline void rk2_step(double y[8], Point &pt, double dl, double a, double h)
+{
+    double A[4], ym[8];
+    geodesic_accel(pt, a, h, y + 4, A);
+    for (int i = 0; i < 4; i++) {
+        ym[i] = y[i] + dl * y[4 + i] * 0.5;
+        ym[4 + i] = y[4 + i] + dl * A[i] * 0.5;
+    }
+    Point pm = make_point(ym[1], ym[2], h);
+    geodesic_accel(pm, a, h, ym + 4, A);
+    for (int i = 0; i < 4; i++) {
+        y[i] = y[i] + dl * ym[4 + i];
+        y[4 + i] = y[4 + i] + dl * A[i];
+    }
+    pt = make_point(y[1], y[2], h);
+}
+
+// Pixel part of initialize_photon (metric.c:650-727): k_mu from impact
+// parameters, raised with the BL metric, converted to KS and to MKS X2.
+__host__ __device__ inline void init_photon(const Camera &cm, double alpha, double beta,
+                                            double y[8])
+{
+    double ll = -alpha * cm.sin_i;
+    double qq = beta * beta + cm.mu0sq * (alpha * alpha - 1.);
+    double sgn = (beta < 0.) ? -1. : ((beta > 0.) ? 1. : 0.);
+    double kd2 = sgn * sqrt(fabs(qq - ll * ll * cm.cot2 + cm.cosc2));
+    double ku0 = cm.g00 * (-1.) + cm.g03 * ll;
+    double ku3 = cm.g33 * ll + cm.g03 * (-1.);
+    double ku2 = cm.g22 * kd2;
+    double ku1 = sqrt((-ku0 * (-1.) - ku2 * kd2 - ku3 * ll) / cm.g11);
+    y[0] = 0.;
+    y[1] = cm.X1;
+    y[2] = cm.X2;
+    y[3] = cm.X3;
+    y[4] = ku0 + cm.t01 * ku1;
+    y[5] = ku1;
+    y[6] = ku2 * cm.thp_inv;
+    y[7] = ku3 + cm.t31 * ku1;
+}
+
+// Interpolated primitives at a point (grmhd.c:35-71, with the 3D extension).
+// prims: 8 floats per cell, cells ordered ((i*n2 + j)*n3 + k).
+template <bool DIM3>
+__host__ __device__ inline void sample_prims(const Params &P, const float4 *__restrict__ prims,
+                                             const double y[8], double v[8])
+{
+    int i = (int)((y[1] - P.startx1) / P.dx1 - 0.5);
+    int j = (int)((y[2] - P.startx2) / P.dx2 - 0.5);
+    double d1, d2;
+    if (i < 0) {
+        i = 0;
+        d1 = 0.;
+    } else if (i > P.n1 - 2) {
+        i = P.n1 - 2;
+        d1 = 1.;
+    } else {
+        d1 = (y[1] - ((i + 0.5) * P.dx1 + P.startx1)) / P.dx1;
+    }
+    if (j < 0) {
+        j = 0;
+        d2 = 0.;
+    } else if (j > P.n2 - 2) {
+        j = P.n2 - 2;
+        d2 = 1.;
+    } else {
+        d2 = (y[2] - ((j + 0.5) * P.dx2 + P.startx2)) / P.dx2;
+    }
+    const double c0 = (1. - d1) * (1. - d2), c1 = (1. - d1) * d2;
+    const double c2 = d1 * (1. - d2), c3 = d1 * d2;
+    if (!DIM3) {
+        const size_t b00 = (size_t)(i * P.n2 + j) * 2;
+        const size_t b10 = b00 + (size_t)P.n2 * 2;
+        float4 p00a = prims[b00], p00b = prims[b00 + 1];
+        float4 p01a = prims[b00 + 2], p01b = prims[b00 + 3];
+        float4 p10a = prims[b10], p10b = prims[b10 + 1];
+        float4 p11a = prims[b10 + 2], p11b = prims[b10 + 3];
+#define RT_BI(f00, f01, f10, f11) \
+    ((double)(f00) * c0 + (double)(f01) * c1 + (double)(f10) * c2 + (double)(f11) * c3)
+        v[0] = RT_BI(p00a.x, p01a.x, p10a.x, p11a.x);
+        v[1] = RT_BI(p00a.y, p01a.y, p10a.y, p11a.y);
+        v[2] = RT_BI(p00a.z, p01a.z, p10a.z, p11a.z);
+        v[3] = RT_BI(p00a.w, p01a.w, p10a.w, p11a.w);
+        v[4] = RT_BI(p00b.x, p01b.x, p10b.x, p11b.x);
+        v[5] = RT_BI(p00b.y, p01b.y, p10b.y, p11b.y);
+        v[6] = RT_BI(p00b.z, p01b.z, p10b.z, p11b.z);
+        v[7] = RT_BI(p00b.w, p01b.w, p10b.w, p11b.w);
+    } else {
+        // periodic in X3 = phi; cell centres at startx3 + (k + 1/2) dx3
+        double period = P.n3 * P.dx3;
+        double ph = fmod(y[3] - P.startx3, period);
+        if (ph < 0.)
+            ph += period;
+        double t = ph / P.dx3 - 0.5;
+        double fk = floor(t);
+        double d3 = t - fk;
+        int k = (int)fk;
+        int k0 = (k + P.n3) % P.n3, k1 = (k + 1) % P.n3;
+        const size_t n3 = P.n3, n2 = P.n2;
+        size_t b00 = ((size_t)i * n2 + j) * n3, b01 = b00 + n3;
+        size_t b10 = b00 + n2 * n3, b11 = b10 + n3;
+        double w[2][8];
+        const int ks[2] = {k0, k1};
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const size_t kk = ks[q];
+            float4 p00a = prims[(b00 + kk) * 2], p00b = prims[(b00 + kk) * 2 + 1];
+            float4 p01a = prims[(b01 + kk) * 2], p01b = prims[(b01 + kk) * 2 + 1];
+            float4 p10a = prims[(b10 + kk) * 2], p10b = prims[(b10 + kk) * 2 + 1];
+            float4 p11a = prims[(b11 + kk) * 2], p11b = prims[(b11 + kk) * 2 + 1];
+            w[q][0] = RT_BI(p00a.x, p01a.x, p10a.x, p11a.x);
+            w[q][1] = RT_BI(p00a.y, p01a.y, p10a.y, p11a.y);
+            w[q][2] = RT_BI(p00a.z, p01a.z, p10a.z, p11a.z);
+            w[q][3] = RT_BI(p00a.w, p01a.w, p10a.w, p11a.w);
+            w[q][4] = RT_BI(p00b.x, p01b.x, p10b.x, p11b.x);
+            w[q][5] = RT_BI(p00b.y, p01b.y, p10b.y, p11b.y);
+            w[q][6] = RT_BI(p00b.z, p01b.z, p10b.z, p11b.z);
+            w[q][7] = RT_BI(p00b.w, p01b.w, p10b.w, p11b.w);
+        }
+#undef RT_BI
+        for (int q = 0; q < 8; q++)
+            v[q] = (1. - d3) * w[0][q] + d3 * w[1][q];
+    }
+}
+
+// The fluid state the transfer needs at one sample.
+struct Fluid {
+    double ne, thetae, B;   // cgs density, dimensionless temperature, gauss
+    double mu;              // cos(pitch angle)
+    double gdop;            // nu_plasma / nu_observed = -(u . k)
+    double dthetae;         // d Thetae / d R_high
+};
+
+// get_fluid_params (raptor_harm_model.c:190-320) + pitch_angle and the
+// plasma-frame frequency factor (radiative_transfer.c:17-47).  Returns false
+// when b^2/rho > 1 (the sample is skipped).
+template <bool DIM3>
+__host__ __device__ inline bool fluid_sample(const Params &P, const float4 *__restrict__ prims,
+                                             const Point &pt, const double y[8], Fluid &F,
+                                             double *ucon_out = nullptr,
+                                             double *bcon_out = nullptr)
+{
+    double v[8];
+    sample_prims<DIM3>(P, prims, y, v);
+    const double rho = v[0], uu = v[1];
+    const double a = P.a;
+    MetricDD g = metric_dd(pt, a, P.hslope);
+    // the two entries of g^{mu nu} the velocity reconstruction uses (metric.c:221-222)
+    double r = pt.r;
+    double irho2 = 1. / (r * r + a * a * pt.c * pt.c);
+    double gcon00 = -1. - 2. * r * irho2;
+    double gcon01 = 2. * irho2;
+    const double V1 = v[2], V2 = v[3], V3 = v[4];
+    double VdotV = g.g11 * V1 * V1 + g.g13 * V1 * V3 + g.g22 * V2 * V2 + g.g13 * V3 * V1 +
+                   g.g33 * V3 * V3;
+    double Vfac = sqrt(-1. / gcon00 * (1. + fabs(VdotV)));
+    double u[4], ud[4], b[4], bd[4];
+    u[0] = -Vfac * gcon00;
+    u[1] = V1 - Vfac * gcon01;
+    u[2] = V2;
+    u[3] = V3;
+    lower(g, u, ud);
+    double UdotBp = ud[1] * v[5] + ud[2] * v[6] + ud[3] * v[7];
+    b[0] = UdotBp;
+    b[1] = (v[5] + u[1] * UdotBp) / u[0];
+    b[2] = (v[6] + u[2] * UdotBp) / u[0];
+    b[3] = (v[7] + u[3] * UdotBp) / u[0];
+    lower(g, b, bd);
+    double Bsq = b[0] * bd[0] + b[1] * bd[1] + b[2] * bd[2] + b[3] * bd[3];
+    F.ne = rho * P.Ne_unit;
+    F.B = sqrt(Bsq) * P.B_unit + kSmall;
+    double beta = uu * (4. / 3. - 1.) / (0.5 * (Bsq + kSmall));
+    double b2 = beta * beta;
+    double trat = P.R_high * b2 / (1. + b2) + P.R_low / (1. + b2);
+    double gam2 = 0.5 * ((1. + 2. / 3. * (trat + 1.) / (trat + 2.)) + 4. / 3.);
+    const double mr = kProtonMass / kElectronMass;
+    F.thetae = (uu / rho) * ((gam2 - 1.) * mr / (1. + trat));
+    double dgam2 = (1. / 3.) / ((trat + 2.) * (trat + 2.));
+    double dunit = mr * (dgam2 * (1. + trat) - (gam2 - 1.)) / ((1. + trat) * (1. + trat));
+    F.dthetae = (uu / rho) * dunit * (b2 / (1. + b2));
+    if (F.thetae < 0.) {
+        F.thetae = kSmall;
+        F.dthetae = 0.;
+    }
+    if (ucon_out) {
+        for (int q = 0; q < 4; q++) {
+            ucon_out[q] = u[q];
+            bcon_out[q] = b[q];
+        }
+    }
+    if (Bsq / rho > 1.0)
+        return false;
+    // pitch angle (cos) and Doppler factor with k lowered by the same metric
+    double kd[4];
+    lower(g, y + 4, kd);
+    double ku = kd[0] * u[0] + kd[1] * u[1] + kd[2] * u[2] + kd[3] * u[3];
+    double mu = (kd[0] * b[0] + kd[1] * b[1] + kd[2] * b[2] + kd[3] * b[3]) /
+                (fabs(ku) * F.B / P.B_unit);
+    if (fabs(mu) > 1.)
+        mu /= fabs(mu);
+    F.mu = mu;
+    F.gdop = -ku;
+    return true;
+}
+
+// Thermal synchrotron emissivity (j_nu.c:130-153) and Kirchhoff absorptivity
+// (j_nu.c:188-200).  X^(1/2) and X^(1/6), X^(1/3) are sqrt and cube roots.
+__host__ __device__ inline void emission(double B, double mu, double Te, double nu_p, double ne,
+                                         double &j, double &an, double &X_out)
+{
+    double sth = sqrt(1. - mu * mu);
+    double nu_c = kElectronCharge * B / (2. * kPi * kElectronMass * kC);
+    double nu_s = (2. / 9.) * nu_c * Te * Te * sth;
+    double X = nu_p / nu_s;
+    X_out = X;
+    double cb = cbrt(X);
+    double sx = sqrt(X);
+    const double c1112 = 1.8877486253633868;   // 2^(11/12)
+    double ff = sx + c1112 * sqrt(cb);
+    ff = ff * ff;
+    if (nu_p == 1. || fabs(sth) < 1.e-10 || !(Te > 1e-4)) {
+        j = 0.;
+    } else {
+        double K2 = 2. * Te * Te;
+        j = ne * sqrt(2.) * kPi * nu_s / (3. * K2) * ff * exp(-cb) * kE2C;
+    }
+    double T = Te * kElectronMass * kC * kC / kBoltzmann;
+    double Bnu = 2. * nu_p * nu_p * nu_p / (kC * kC) * 1. /
+                 (exp(kPlanck * nu_p / (kBoltzmann * T)) - 1.);
+    an = j * kE2CH / Bnu / kE2C;
+}
+
+// d ln j / d ln X of the fit at fixed n_e and nu_s
+__host__ __device__ inline double dlnj_dlnX(double X)
+{
+    const double c1112 = 1.8877486253633868;
+    double sx = sqrt(X), cb = cbrt(X), t = sqrt(cb);
+    return 2. * (0.5 * sx + c1112 / 6. * t) / (sx + c1112 * t) - cb / 3.;
+}
+
+// Per-ray accumulators for NF frequencies.
+template <int NF, bool GRAD>
+struct Accum {
+    double tau[NF];
+    double I[NF];
+    double dtau[GRAD ? NF : 1][2];
+    double dI[GRAD ? NF : 1][2];
+    __host__ __device__ void clear()
+    {
+        for (int f = 0; f < NF; f++) {
+            tau[f] = 0.;
+            I[f] = 0.;
+            if (GRAD) {
+                dtau[f][0] = dtau[f][1] = 0.;
+                dI[f][0] = dI[f][1] = 0.;
+            }
+        }
+    }
+};
+
+// radiative_transfer.c:71-98 for one sample: end-point quadrature with the
+// step's |dlambda|, tau gate 0 <= tau < ln 1000.
+template <int NF, bool GRAD>
+__host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F, double dlabs,
+                                                Accum<NF, GRAD> &acc)
+{
+    const double lnth = 6.907755278982137;   // log(1000.)
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        double nu_p = F.gdop * P.freq[f];
+        if (nu_p <= 0.)
+            nu_p = 1.;
+        double j, an, X;
+        emission(F.B, F.mu, F.thetae, nu_p, F.ne, j, an, X);
+        double dlf = dlabs * P.dl_scale[f];
+        double dtau = nu_p * an * dlf;
+        acc.tau[f] += dtau;
+        double dI = (j / (nu_p * nu_p)) * exp(-acc.tau[f]) * dlf;
+        double dlnj0 = 0., dlnj1 = 0., dlna1 = 0.;
+        if (GRAD && j != 0.) {
+            double Dx = dlnj_dlnX(X);
+            double dlnB = 0.5 * (F.B - kSmall) / F.B;
+            double dlnT = F.dthetae / F.thetae;
+            double xx = kPlanck * nu_p / (F.thetae * kElectronMass * kC * kC);
+            double dlnBnu = xx / (-expm1(-xx));
+            dlnj0 = 1. + dlnB * (1. - Dx);
+            dlnj1 = -2. * Dx * dlnT;
+            dlna1 = dlnj1 - dlnBnu * dlnT;
+        }
+        if (GRAD) {
+            if (dtau != 0.) {
+                acc.dtau[f][0] += dtau * dlnj0;
+                acc.dtau[f][1] += dtau * dlna1;
+            }
+        }
+        if (acc.tau[f] < lnth && !(acc.tau[f] < 0.)) {
+            acc.I[f] += dI;
+            if (GRAD && dI != 0.) {
+                acc.dI[f][0] += dI * (dlnj0 - acc.dtau[f][0]);
+                acc.dI[f][1] += dI * (dlnj1 - acc.dtau[f][1]);
+            }
+        }
+    }
+}
+
+// A ray being traced.
+template <int NF, bool GRAD>
+struct Ray {
+    double y[8];
+    Point pt;
+    double thetadot_prev;
+    double beta;
+    int steps, turns, samples;
+    bool terminate;
+    Accum<NF, GRAD> acc;
+};
+
+__host__ __device__ inline void pixel_impact(const Params &P, int pix, double &alpha,
+                                             double &beta)
+{
+    int yy = pix / P.width, x = pix - yy * P.width;
+    double stepx = P.cam_x / (double)P.width;
+    double stepy = P.cam_y / (double)P.height;
+    alpha = -P.cam_x * 0.5 + (x + 0.5) * stepx;
+    beta = -P.cam_y * 0.5 + (yy + 0.5) * stepy;
+}
+
+template <int NF, bool GRAD>
+__host__ __device__ inline void ray_init(const Params &P, int cam, int pix, Ray<NF, GRAD> &R)
+{
+    double alpha, beta;
+    pixel_impact(P, pix, alpha, beta);
+    init_photon(P.cam[cam], alpha, beta, R.y);
+    R.pt = make_point(R.y[1], R.y[2], P.hslope);
+    R.thetadot_prev = 0.;
+    R.beta = beta;
+    R.steps = 0;
+    R.turns = 0;
+    R.samples = 0;
+    R.terminate = false;
+    R.acc.clear();
+}
+
+// Loop condition of integrate_geodesic (integrator.c:228-229).
+template <int NF, bool GRAD>
+__host__ __device__ inline bool ray_alive(const Params &P, const Ray<NF, GRAD> &R)
+{
+    return R.pt.r < kCutoffOuter && R.pt.r > P.cutoff_inner && R.steps < P.max_steps &&
+           !R.terminate;
+}
+
+// One iteration of integrate_geodesic's loop (integrator.c:243-284).
+// Returns false when the ray escaped through the outer grid edge (the
+// `break` at integrator.c:271-272).
+template <int NF, bool GRAD, bool DIM3>
+__host__ __device__ inline bool ray_step(const Params &P, const float4 *__restrict__ prims,
+                                         Ray<NF, GRAD> &R)
+{
+    if (R.thetadot_prev * R.y[6] < 0. && R.steps > 2)
+        R.turns += 1;
+    R.thetadot_prev = R.y[6];
+    if ((R.beta < 0. && R.turns > kMaxOrder) || (R.beta > 0. && R.turns > kMaxOrder + 1))
+        R.terminate = true;
+    double dl = stepsize(R.y, P.stepsize);
+    rk2_step(R.y, R.pt, dl, P.a, P.hslope);
+    if (R.y[1] > P.stopx1 && R.y[5] < 0.)
+        return false;
+    if (R.y[1] < P.stopx1 && R.y[1] > P.startx1 && R.y[2] > P.startx2 && R.y[2] < P.stopx2) {
+        Fluid F;
+        if (fluid_sample<DIM3>(P, prims, R.pt, R.y, F)) {
+            R.samples++;
+            transfer_sample<NF, GRAD>(P, F, fabs(dl), R.acc);
+        }
+    }
+    R.steps++;
+    return true;
+}
+
+// Host set-up.
+// newtonraphson.c:49-72 solved to a fixed point (the root is unique)
+inline double theta_to_x2(double theta, double h)
+{
+    double x = theta / kPi, prev = -1.;
+    for (int it = 0; it < 200 && x != prev; it++) {
+        prev = x;
+        double f = kPi * x + 0.5 * (1. - h) * std::sin(2. * kPi * x) - theta;
+        double fp = kPi + kPi * (1. - h) * std::cos(2. * kPi * x);
+        x = x - f / fp;
+        if (std::fabs(x - prev) <= 1e-15 * std::fabs(x))
+            break;
+    }
+    return std::fmin(1., std::fmax(x, 0.));
+}
+
+// metric.c:650-692 (pixel-independent part) and GRmath.c:135-184
+inline Camera make_camera(double incl_deg, double a, double h)
+{
+    Camera c;
+    double mu0 = std::cos(incl_deg / 180. * kPi);
+    double thc = std::acos(mu0);
+    double sinc2 = std::sin(thc) * std::sin(thc);
+    double cosc2 = std::cos(thc) * std::cos(thc);
+    c.sin_i = std::sqrt(1. - mu0 * mu0);
+    c.mu0sq = mu0 * mu0;
+    c.cot2 = cosc2 / sinc2;
+    c.cosc2 = cosc2;
+    double r = kRcam;
+    double st = std::sin(thc), ct = std::cos(thc);
+    double sigma = r * r + a * a * ct * ct;
+    double delta = r * r + a * a - 2. * r;
+    double A_ = (r * r + a * a) * (r * r + a * a) - delta * a * a * st * st;
+    c.g11 = sigma / delta * r * r;
+    c.g00 = -A_ / (sigma * delta);
+    c.g03 = -2. * a * r / (sigma * delta);
+    c.g33 = (delta - a * a * st * st) / (sigma * delta * st * st);
+    c.g22 = 1. / sigma;
+    double X1 = std::log(kRcam);
+    double rc = std::exp(X1);
+    double dc = rc * rc - 2. * rc + a * a;
+    c.t01 = 2. * rc / dc * rc;
+    c.t31 = a / dc * rc;
+    double rp = 1. + std::sqrt(1. - a * a), rm = 1. - std::sqrt(1. - a * a);
+    double lg = std::log((rc - rp) / (rc - rm));
+    c.X1 = X1;
+    c.X3 = 90. / 180. * kPi + (a / (2. * std::sqrt(1. - a * a)) * lg);
+    c.X2 = theta_to_x2(thc, h);
+    c.thp_inv = 1. / (kPi * (1. + (1. - h) * std::cos(2. * kPi * c.X2)));
+    return c;
+}
+
+
+// Host-side set-up of a launch from the C-ABI structs (grmhd.c:17-33 units,
+// utilities.c:17-21 cutoff).  Returns NULL or an error message.
+template <class RtParams, class RtGrid>
+inline const char *fill_params(const RtParams &p, const RtGrid &g, const double *incl, int ncam,
+                               const double *freqs, int nf, Params &P, double nu3[kMaxFreq])
+{
+    memset(&P, 0, sizeof P);
+    P.n1 = g.n1;
+    P.n2 = g.n2;
+    P.n3 = g.n3;
+    P.startx1 = g.startx1;
+    P.startx2 = g.startx2;
+    P.startx3 = g.startx3;
+    P.dx1 = g.dx1;
+    P.dx2 = g.dx2;
+    P.dx3 = g.dx3;
+    P.stopx1 = g.startx1 + g.n1 * g.dx1;
+    P.stopx2 = g.startx2 + g.n2 * g.dx2;
+    P.a = g.a;
+    P.hslope = g.hslope;
+    P.width = p.width;
+    P.height = p.height;
+    P.npix = p.width * p.height;
+    P.cam_x = p.cam_size_x;
+    P.cam_y = p.cam_size_y;
+    P.stepsize = p.stepsize;
+    P.max_steps = p.max_steps;
+    P.cutoff_inner = (1. + std::sqrt(1. - g.a * g.a)) * (1. + kHorizonMarg);
+    // grmhd.c:17-33
+    double L_unit = kGGrav * p.MBH / (kC * kC);
+    double RHO_unit = p.M_unit / std::pow(L_unit, 3);
+    P.B_unit = kC * std::sqrt(4. * kPi * RHO_unit);
+    P.Ne_unit = RHO_unit / (kProtonMass + kElectronMass);
+    P.Rg = kGGrav * p.MBH / kC / kC;
+    P.R_high = p.R_high;
+    P.R_low = p.R_low;
+    P.nf = nf;
+    for (int f = 0; f < kMaxFreq; f++) {
+        double nu = freqs[std::min(f, nf - 1)];
+        if (!(nu > 0.))
+            return "frequencies must be positive";
+        P.freq[f] = nu;
+        P.dl_scale[f] = P.Rg / nu;
+        nu3[f] = std::pow(nu, 3.);
+    }
+    P.ncam = ncam;
+    for (int i = 0; i < ncam; i++) {
+        if (!(incl[i] > 0. && incl[i] < 180.))
+            return "inclinations must lie in (0, 180) degrees";
+        P.cam[i] = make_camera(incl[i], g.a, g.hslope);
+    }
+    return nullptr;
+}
+
+}  // namespace rt

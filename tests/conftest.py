@@ -1,0 +1,47 @@
+import os
+import sys
+import warnings
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+
+
+# The synthetic workload of the parity tests: a seeded Fishbone-Moncrief torus
+# (a = 0.9375, h = 0.3, Rout = 40, gamma = 13/9), M_unit 1e23 g, R_high 10,
+# R_low 1, 230 GHz, inclination 60 deg, 40 M field of view.
+M_UNIT = 1e23
+R_HIGH = 10.0
+FREQ = 2.3e11
+INCL = 60.0
+
+
+@pytest.fixture(scope="session")
+def dump128():
+    from raptor_amd import torus
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        return torus.fishbone_moncrief(128, 128)
+
+
+@pytest.fixture(scope="session")
+def dump256():
+    from raptor_amd import torus
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        return torus.fishbone_moncrief(256, 256)
+
+
+def rel_err(a, b, floor_frac=1e-6):
+    """Per-pixel relative error with an absolute floor floor_frac*max|b|."""
+    a = np.asarray(a, float)
+    b = np.asarray(b, float)
+    floor = floor_frac * np.max(np.abs(b))
+    return np.abs(a - b) / np.maximum(np.abs(b), floor)

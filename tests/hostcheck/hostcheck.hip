@@ -1,0 +1,71 @@
+// hostcheck.hip -- test-only CPU build of the tracer's per-ray physics.
+//
+// The __host__ __device__ functions of raptor_amd/csrc/rt_physics.hpp are
+// compiled for the host here and driven ray by ray on the CPU, so that the
+// not-gpu test suite can compare the kernel's arithmetic with the oracle
+// before any GPU run.  This library is test infrastructure: the product
+// (libraptor_rt.so) has no CPU path.
+#include "../../include/raptor_rt.h"
+#include "../../raptor_amd/csrc/rt_physics.hpp"
+
+#include <vector>
+
+using namespace rt;
+
+template <int NF, bool GRAD>
+static void run(const Params &P, const float4 *prims, const double *nu3, int total, double *img,
+                double *dimg, int *cnt)
+{
+    for (int ray = 0; ray < total; ray++) {
+        Ray<NF, GRAD> R;
+        ray_init<NF, GRAD>(P, ray / P.npix, ray % P.npix, R);
+        bool dim3 = P.n3 > 1;
+        while (ray_alive<NF, GRAD>(P, R)) {
+            bool cont = dim3 ? ray_step<NF, GRAD, true>(P, prims, R)
+                             : ray_step<NF, GRAD, false>(P, prims, R);
+            if (!cont)
+                break;
+        }
+        for (int f = 0; f < P.nf; f++) {
+            img[(size_t)ray * P.nf + f] = R.acc.I[f] * nu3[f];
+            if (GRAD) {
+                dimg[((size_t)ray * P.nf + f) * 2] = R.acc.dI[f][0] * nu3[f];
+                dimg[((size_t)ray * P.nf + f) * 2 + 1] = R.acc.dI[f][1] * nu3[f];
+            }
+        }
+        if (cnt) {
+            cnt[2 * ray] = R.steps;
+            cnt[2 * ray + 1] = R.samples;
+        }
+    }
+}
+
+extern "C" int hc_render(const rt_params *p, const rt_grid *g, const float *soa,
+                         const double *incl, int ncam, const double *freqs, int nf, double *img,
+                         double *dimg, int *cnt)
+{
+    Params *P = new Params;
+    double nu3[kMaxFreq];
+    const char *msg = fill_params(*p, *g, incl, ncam, freqs, nf, *P, nu3);
+    if (msg) {
+        delete P;
+        return -1;
+    }
+    size_t ncell = (size_t)g->n1 * g->n2 * g->n3;
+    std::vector<float4> aos(2 * ncell);
+    for (size_t c = 0; c < ncell; c++) {
+        aos[2 * c] = make_float4(soa[c], soa[ncell + c], soa[2 * ncell + c], soa[3 * ncell + c]);
+        aos[2 * c + 1] = make_float4(soa[4 * ncell + c], soa[5 * ncell + c], soa[6 * ncell + c],
+                                     soa[7 * ncell + c]);
+    }
+    int total = ncam * P->npix;
+    if (nf == 1) {
+        if (dimg) run<1, true>(*P, aos.data(), nu3, total, img, dimg, cnt);
+        else run<1, false>(*P, aos.data(), nu3, total, img, dimg, cnt);
+    } else {
+        if (dimg) run<16, true>(*P, aos.data(), nu3, total, img, dimg, cnt);
+        else run<16, false>(*P, aos.data(), nu3, total, img, dimg, cnt);
+    }
+    delete P;
+    return 0;
+}

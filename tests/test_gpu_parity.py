@@ -1,0 +1,273 @@
+"""Parity of the HIP path (through the C-ABI) with the FP64 CPU oracle.
+
+Bar (north star): total flux within 1e-4 relative, every pixel within 1e-3
+relative with an absolute floor of 1e-6 * max(I); building blocks to
+near-FP64 rounding.
+"""
+import numpy as np
+import pytest
+
+from conftest import FREQ, INCL, M_UNIT, R_HIGH, rel_err
+from oracle import oracle as O
+from raptor_amd import rt
+
+pytestmark = pytest.mark.gpu
+
+FLUX_TOL = 1e-4
+PIX_TOL = 1e-3
+
+
+@pytest.fixture(scope="module")
+def tracer():
+    tr = rt.Tracer(0)
+    yield tr
+    tr.close()
+
+
+def _setup(tr, dump, W, H=None, **kw):
+    H = W if H is None else H
+    tr.set_params(width=W, height=H, M_unit=kw.get("M_unit", M_UNIT),
+                  R_high=kw.get("R_high", R_HIGH), R_low=kw.get("R_low", 1.0))
+    tr.load_dump(dump)
+    return O.params(dump.grid, width=W, height=H, inclination=kw.get("incl", INCL),
+                    M_unit=kw.get("M_unit", M_UNIT), R_high=kw.get("R_high", R_HIGH),
+                    R_low=kw.get("R_low", 1.0))
+
+
+def _report(name, gpu, ora):
+    r = rel_err(gpu, ora)
+    worst = np.argsort(r)[-5:][::-1]
+    print("%s: flux rel %.3e, max pix rel %.3e, worst %s" %
+          (name, gpu.sum() / ora.sum() - 1, r.max(), [(int(i), float(r[i])) for i in worst]))
+    return r
+
+
+def test_c0_image_matches_oracle(tracer, dump128):
+    """64x64 image of the 128^2 torus at 230 GHz, i = 60 deg (the CPU config)."""
+    P = _setup(tracer, dump128, 64)
+    img, spec = tracer.render([INCL], [FREQ])
+    ora = O.render(P, dump128.prims, [FREQ])[:, 0]
+    gpu = img[0, :, 0]
+    r = _report("c0", gpu, ora)
+    assert abs(gpu.sum() / ora.sum() - 1) < FLUX_TOL
+    assert r.max() < PIX_TOL
+    assert spec[0, 0] == pytest.approx(gpu.sum(), rel=1e-14)
+
+
+@pytest.mark.parametrize("W,H,incl", [(37, 23, 60.0), (1, 1, 30.0), (16, 16, 1.0),
+                                      (16, 16, 179.0), (24, 8, 90.0)])
+def test_ragged_sizes_and_inclinations(tracer, dump128, W, H, incl):
+    P = _setup(tracer, dump128, W, H, incl=incl)
+    img, _ = tracer.render([incl], [FREQ])
+    ora = O.render(P, dump128.prims, [FREQ])[:, 0]
+    if ora.max() == 0:
+        assert np.all(img == 0)
+        return
+    r = _report("%dx%d i=%g" % (W, H, incl), img[0, :, 0], ora)
+    assert r.max() < PIX_TOL
+
+
+def test_ray_counters_match_oracle(tracer, dump128):
+    W = 32
+    P = _setup(tracer, dump128, W)
+    n = W * W
+    d_img = tracer.alloc(n * 8)
+    d_cnt = tracer.alloc(n * 8)
+    try:
+        tracer.render_device([INCL], [FREQ], d_img, None, d_cnt)
+        tracer.synchronize()
+        cnt = tracer.d2h(d_cnt, (n, 2), np.int32)
+    finally:
+        tracer.free(d_img)
+        tracer.free(d_cnt)
+    _, oc = O.render(P, dump128.prims, [FREQ], counters=True)
+    assert (cnt[:, 0] == oc[:, 0]).mean() > 0.99
+    assert (cnt[:, 1] == oc[:, 2]).mean() > 0.99
+
+
+def test_gradients_match_oracle(tracer, dump128):
+    P = _setup(tracer, dump128, 32)
+    img, dimg, _ = tracer.render([INCL], [FREQ], grad=True)
+    oimg, odimg = O.render(P, dump128.prims, [FREQ], grad=True)
+    assert rel_err(img[0, :, 0], oimg[:, 0]).max() < PIX_TOL
+    for q in range(2):
+        g, o = dimg[0, :, 0, q], odimg[:, 0, q]
+        assert abs(g.sum() / o.sum() - 1) < 1e-4
+        assert rel_err(g, o, 1e-5).max() < 1e-3
+
+
+def test_multi_frequency_and_camera_batch(tracer, dump128):
+    """One launch over 3 inclinations x 3 frequencies equals the per-image launches."""
+    _setup(tracer, dump128, 24)
+    incls = [20.0, 60.0, 85.0]
+    freqs = [8.6e10, 2.3e11, 6.9e11]
+    img, spec = tracer.render(incls, freqs)
+    for c, inc in enumerate(incls):
+        for f, nu in enumerate(freqs):
+            one, _ = tracer.render([inc], [nu])
+            np.testing.assert_allclose(img[c, :, f], one[0, :, 0], rtol=1e-12, atol=0)
+    P = O.params(dump128.grid, width=24, height=24, inclination=85.0, M_unit=M_UNIT,
+                 R_high=R_HIGH)
+    ora = O.render(P, dump128.prims, freqs)
+    for f in range(3):
+        assert rel_err(img[2, :, f], ora[:, f]).max() < PIX_TOL
+
+
+def test_repeat_runs_are_bitwise_identical(tracer, dump128):
+    _setup(tracer, dump128, 48)
+    a, _ = tracer.render([INCL], [FREQ])
+    b, _ = tracer.render([INCL], [FREQ])
+    assert np.array_equal(a, b)
+
+
+def test_3d_phi_independent_dump_equals_2d(tracer, dump128):
+    """A 3D dump whose phi slices are identical renders the 2D image."""
+    from raptor_amd.torus import Dump
+    import dataclasses
+    _setup(tracer, dump128, 32)
+    img2, _ = tracer.render([INCL], [FREQ])
+    g3 = dataclasses.replace(dump128.grid, n3=6, dx3=2 * np.pi / 6)
+    d3 = Dump(grid=g3, prims=np.repeat(dump128.prims, 6, axis=3))
+    tracer.load_dump(d3)
+    img3, _ = tracer.render([INCL], [FREQ])
+    assert rel_err(img3[0, :, 0], img2[0, :, 0]).max() < 1e-9
+
+
+def test_3d_dump_matches_oracle(tracer):
+    from raptor_amd import torus
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        d3 = torus.fishbone_moncrief(48, 32, 16)
+    P = _setup(tracer, d3, 32)
+    img, _ = tracer.render([INCL], [FREQ])
+    ora = O.render(P, d3.prims, [FREQ])[:, 0]
+    r = _report("3d", img[0, :, 0], ora)
+    assert abs(img[0, :, 0].sum() / ora.sum() - 1) < FLUX_TOL
+    assert r.max() < PIX_TOL
+
+
+def test_c1_full_size_sample(tracer, dump256):
+    """1024^2 image of the 256^2 dump: a pixel sample against the oracle and
+    size-independent properties (finite, non-negative, deterministic flux)."""
+    W = 1024
+    P = _setup(tracer, dump256, W)
+    img, spec = tracer.render([INCL], [FREQ])
+    gpu = img[0, :, 0]
+    assert np.all(np.isfinite(gpu)) and gpu.min() >= 0
+    rng = np.random.default_rng(7)
+    pix = np.sort(rng.choice(W * W, 1024, replace=False)).astype(np.int32)
+    ora = O.render(P, dump256.prims, [FREQ], pixels=pix)[:, 0]
+    r = np.abs(gpu[pix] - ora) / np.maximum(np.abs(ora), 1e-6 * gpu.max())
+    assert r.max() < PIX_TOL
+    img2, spec2 = tracer.render([INCL], [FREQ])
+    assert spec2[0, 0] == spec[0, 0]
+
+
+# ---- building blocks ------------------------------------------------------
+
+
+def _rand_points(rng, n, grid):
+    X = np.zeros((n, 4))
+    X[:, 1] = rng.uniform(np.log(1.5), np.log(1e4), n)
+    X[:, 2] = rng.uniform(0.02, 0.98, n)
+    X[:, 3] = rng.uniform(0, 2 * np.pi, n)
+    return X
+
+
+def test_probe_metric(tracer, dump128):
+    P = _setup(tracer, dump128, 8)
+    rng = np.random.default_rng(1)
+    X = _rand_points(rng, 200, dump128.grid)
+    out = tracer.probe(rt.PROBE_METRIC, X)
+    for i in range(len(X)):
+        np.testing.assert_allclose(out[i, :16].reshape(4, 4), O.metric_dd(P, X[i]),
+                                   rtol=1e-13, atol=1e-13)
+        np.testing.assert_allclose(out[i, 16:].reshape(4, 4), O.metric_uu(P, X[i]),
+                                   rtol=1e-13, atol=1e-15)
+
+
+def test_probe_accel_and_rk2(tracer, dump128):
+    P = _setup(tracer, dump128, 8)
+    rng = np.random.default_rng(2)
+    X = _rand_points(rng, 100, dump128.grid)
+    K = rng.normal(size=(100, 4))
+    acc = tracer.probe(rt.PROBE_ACCEL, np.hstack([X, K]))
+    y = np.hstack([X, K])
+    step = tracer.probe(rt.PROBE_RK2, y)
+    for i in range(len(X)):
+        G = O.connection(P, X[i])
+        A = -np.einsum("mab,a,b->m", G, K[i], K[i])
+        np.testing.assert_allclose(acc[i], A, rtol=1e-11, atol=1e-11 * np.abs(A).max())
+        dl = O.stepsize(P, X[i], K[i])
+        assert step[i, 0] == pytest.approx(dl, rel=1e-14)
+        np.testing.assert_allclose(step[i, 1:], O.rk2_step(P, y[i], dl), rtol=1e-11,
+                                   atol=1e-12)
+
+
+def test_probe_camera(tracer, dump128):
+    _setup(tracer, dump128, 8)
+    rng = np.random.default_rng(3)
+    pts = [(a, b, i) for a, b, i in zip(rng.uniform(-20, 20, 50), rng.uniform(-20, 20, 50),
+                                        rng.choice([10.0, 60.0, 85.0], 50))]
+    out = tracer.probe(rt.PROBE_CAMERA, np.array(pts))
+    for k, (a, b, i) in enumerate(pts):
+        P = O.params(dump128.grid, width=8, height=8, inclination=i, M_unit=M_UNIT,
+                     R_high=R_HIGH)
+        ref = O.init_photon(P, a, b)
+        np.testing.assert_allclose(out[k, 1:], ref[1:], rtol=1e-12, atol=1e-16)
+
+
+def test_probe_emission(tracer):
+    rng = np.random.default_rng(4)
+    n = 300
+    q = np.column_stack([10 ** rng.uniform(-1, 3, n), rng.uniform(-1, 1, n),
+                         10 ** rng.uniform(-5, 2, n), 10 ** rng.uniform(9, 13, n),
+                         10 ** rng.uniform(2, 8, n)])
+    out = tracer.probe(rt.PROBE_EMISSION, q)
+    lib = O.lib()
+    for i in range(n):
+        j = lib.or_j_thsynch(q[i, 0], q[i, 1], q[i, 2], q[i, 3], q[i, 4])
+        a = lib.or_a_th(j, q[i, 3], q[i, 2])
+        assert out[i, 0] == pytest.approx(j, rel=1e-12, abs=1e-300)
+        if np.isfinite(a):
+            assert out[i, 1] == pytest.approx(a, rel=1e-10, abs=1e-300)
+
+
+def test_fluid_params_hook(tracer, dump128):
+    P = _setup(tracer, dump128, 8)
+    rng = np.random.default_rng(5)
+    g = dump128.grid
+    X = np.zeros((300, 4))
+    X[:, 1] = rng.uniform(g.startx1, g.stopx1, 300)
+    X[:, 2] = rng.uniform(g.startx2, g.stopx2, 300)
+    out = tracer.fluid_params(X)
+    for i in range(len(X)):
+        ref = O.fluid_at(P, dump128.prims, X[i])
+        assert bool(out[i, 11]) == ref["ok"]
+        np.testing.assert_allclose(out[i, 3:7], ref["ucon"], rtol=1e-11, atol=1e-14)
+        np.testing.assert_allclose(out[i, 7:11], ref["bcon"], rtol=1e-9, atol=1e-14)
+        if ref["ok"]:
+            assert out[i, 0] == pytest.approx(ref["ne"], rel=1e-12)
+            assert out[i, 1] == pytest.approx(ref["thetae"], rel=1e-10)
+            assert out[i, 2] == pytest.approx(ref["B"], rel=1e-9)
+
+
+# ---- error paths ----------------------------------------------------------
+
+
+def test_error_codes(dump128):
+    with rt.Tracer(0) as tr:
+        with pytest.raises(rt.RtError, match="rt_set_params"):
+            tr.width, tr.height = 4, 4
+            tr.render([60.0], [FREQ])
+        tr.set_params(width=4, height=4, M_unit=M_UNIT, R_high=R_HIGH)
+        with pytest.raises(rt.RtError, match="rt_load_prims"):
+            tr.render([60.0], [FREQ])
+        tr.load_dump(dump128)
+        with pytest.raises(rt.RtError, match="n_freq"):
+            tr.render([60.0], np.full(17, FREQ))
+        with pytest.raises(rt.RtError, match="inclination"):
+            tr.render([0.0], [FREQ])
+        with pytest.raises(rt.RtError):
+            tr.set_params(width=0, height=4, M_unit=M_UNIT, R_high=R_HIGH)
