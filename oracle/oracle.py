@@ -68,6 +68,8 @@ def lib():
         L.or_render.argtypes = [P, _dp, ctypes.c_int, _dp, _ip, ctypes.c_int, _dp, _dp, _ip,
                                 ctypes.c_int]
         L.or_render.restype = ctypes.c_int
+        L.or_transfer_step.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_double, _dp, _dp]
         _LIB = L
     return _LIB
 

@@ -529,6 +529,21 @@ static double or_dlnj_dlnX(double X)
     return 2. * (0.5 * s + c / 6. * t) / (s + c * t) - pow(X, 1. / 3.) / 3.;
 }
 
+/* radiative_transfer.c:88-96: one sample of the end-point quadrature.
+ * tau accumulates dtau = nu_p*alpha*dl; dI = j/nu_p^2 exp(-tau) dl is added
+ * only while 0 <= tau < ln 1000 (a NaN tau closes the gate for good). */
+void or_transfer_step(double *tau, double *I, double j, double an, double nu_p, double dlf,
+                      double *dtau_out, double *dI_out)
+{
+    double dtau = (nu_p * an * dlf);
+    *tau += dtau;
+    double dI = (j / (nu_p * nu_p)) * exp(-*tau) * dlf;
+    if (*tau < log(1000.) && !(*tau < 0.0))
+        *I += dI;
+    *dtau_out = dtau;
+    *dI_out = dI;
+}
+
 /* ------------------------------------------------------------ one ray */
 /* integrator.c:149-289 integrate_geodesic + radiative_transfer.c:49-119.
  * I[f] returns the invariant intensity I_nu/nu^3 before the nu^3 scaling of
@@ -616,9 +631,8 @@ void or_trace_ray(const or_params *P, const double *prims, int x, int yy, int nf
                     double j = or_j_thsynch(F.B, mu, F.thetae, nu_p, F.ne);
                     double an = or_a_th(j, nu_p, F.thetae);
                     double dlf = dlabs * D.Rg / freqs[f];
-                    double dtau = nu_p * an * dlf;
-                    tau[f] += dtau;
-                    double dIk = (j / (nu_p * nu_p)) * exp(-tau[f]) * dlf;
+                    double dtau, dIk;
+                    or_transfer_step(&tau[f], &I[f], j, an, nu_p, dlf, &dtau, &dIk);
                     /* tangents: d ln j and d ln alpha per parameter */
                     double dlnj[2] = {0., 0.}, dlna[2] = {0., 0.};
                     if (dI && j != 0.) {
@@ -639,12 +653,9 @@ void or_trace_ray(const or_params *P, const double *prims, int x, int yy, int nf
                         for (int q = 0; q < 2; q++)
                             dtau_t[f][q] += (dtau != 0.) ? dtau * dlna[q] : 0.;
                     }
-                    if (tau[f] < lnth && !(tau[f] < 0.0)) {
-                        I[f] += dIk;
-                        if (dI) {
-                            for (int q = 0; q < 2; q++)
-                                dI[2 * f + q] += (dIk != 0.) ? dIk * (dlnj[q] - dtau_t[f][q]) : 0.;
-                        }
+                    if (dI && tau[f] < lnth && !(tau[f] < 0.0)) {
+                        for (int q = 0; q < 2; q++)
+                            dI[2 * f + q] += (dIk != 0.) ? dIk * (dlnj[q] - dtau_t[f][q]) : 0.;
                     }
                 }
             }

@@ -54,8 +54,9 @@ def write_harm_ascii(path, dump: Dump, t=0.0):
         cols[:, 33] = ex["gdet"].ravel()
     with open(path, "w") as fh:
         fh.write(" ".join(_fmt_hdr(v) for v in hdr) + "\n")
-        # %.9g round-trips a float32 exactly; the primitives are float32 values
-        np.savetxt(fh, cols, fmt="%.9g")
+        # %.17g round-trips every double; the primitives are float32 values, so an
+        # FP64 reader and an FP32 reader see exactly the same numbers
+        np.savetxt(fh, cols, fmt="%.17g")
 
 
 def _fmt_hdr(v):

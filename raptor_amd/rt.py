@@ -107,8 +107,8 @@ class Tracer:
         image, spectrum = tr.render([60.], [2.3e11])
     """
 
-    def __init__(self, device=0):
-        self.lib = load_library()
+    def __init__(self, device=0, lib_path=LIB_PATH):
+        self.lib = load_library(lib_path)
         self.ctx = self.lib.rt_create(device)
         if not self.ctx:
             raise RtError("rt_create failed: %s" % self.lib.rt_last_error(None).decode())

@@ -1,0 +1,79 @@
+"""CPU build of the tracer's per-ray physics (rt_physics.hpp compiled for the
+host, tests/hostcheck) against the oracle.  This checks the kernel's
+arithmetic before any GPU run; the GPU path itself is checked in
+test_gpu_parity.py through the C-ABI."""
+import ctypes
+import os
+import subprocess
+import warnings
+
+import numpy as np
+import pytest
+
+from conftest import FREQ, INCL, M_UNIT, R_HIGH, rel_err
+from oracle import oracle as O
+from raptor_amd import rt
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "hostcheck", "hostcheck.hip")
+LIB = os.path.join(HERE, "hostcheck", "libhostcheck.so")
+PHYS = os.path.join(os.path.dirname(HERE), "raptor_amd", "csrc", "rt_physics.hpp")
+
+
+def _lib():
+    newest = max(os.path.getmtime(SRC), os.path.getmtime(PHYS))
+    if not os.path.exists(LIB) or os.path.getmtime(LIB) < newest:
+        hipcc = "/opt/rocm/bin/hipcc"
+        if not os.path.exists(hipcc):
+            pytest.skip("hipcc not available to build the host check")
+        subprocess.run([hipcc, "--offload-arch=gfx950", "-O2", "-std=c++17", "-fPIC", "-shared",
+                        "-o", LIB, SRC], check=True, capture_output=True)
+    return ctypes.CDLL(LIB)
+
+
+def _render(dump, W, incl, freqs, grad=False, R_high=R_HIGH):
+    L = _lib()
+    g = dump.grid
+    grid = rt.RtGrid(n1=g.n1, n2=g.n2, n3=g.n3, startx1=g.startx1, startx2=g.startx2,
+                     startx3=g.startx3, dx1=g.dx1, dx2=g.dx2, dx3=g.dx3, a=g.a, hslope=g.hslope,
+                     R0=g.R0)
+    p = rt.RtParams(MBH=8.9505e39, M_unit=M_UNIT, R_high=R_high, R_low=1.0, width=W, height=W,
+                    cam_size_x=40.0, cam_size_y=40.0, stepsize=0.03, max_steps=100000)
+    prims = np.ascontiguousarray(dump.prims, np.float32)
+    incl = np.ascontiguousarray(incl, float)
+    fr = np.ascontiguousarray(freqs, float)
+    n = incl.size * W * W
+    img = np.zeros(n * fr.size)
+    dimg = np.zeros(2 * n * fr.size) if grad else None
+    cnt = np.zeros(2 * n, np.int32)
+    vp = ctypes.c_void_p
+    rc = L.hc_render(ctypes.byref(p), ctypes.byref(grid), vp(prims.ctypes.data),
+                     vp(incl.ctypes.data), incl.size, vp(fr.ctypes.data), fr.size,
+                     vp(img.ctypes.data), vp(dimg.ctypes.data) if grad else None,
+                     vp(cnt.ctypes.data))
+    assert rc == 0
+    return img.reshape(n, fr.size), dimg, cnt.reshape(n, 2)
+
+
+def test_host_build_matches_oracle_image_and_counters(dump128):
+    W = 32
+    img, dimg, cnt = _render(dump128, W, [INCL], [FREQ], grad=True)
+    P = O.params(dump128.grid, width=W, height=W, inclination=INCL, M_unit=M_UNIT,
+                 R_high=R_HIGH)
+    oimg, odimg, ocnt = O.render(P, dump128.prims, [FREQ], grad=True, counters=True)
+    assert rel_err(img[:, 0], oimg[:, 0]).max() < 1e-8
+    assert np.array_equal(cnt[:, 0], ocnt[:, 0]) and np.array_equal(cnt[:, 1], ocnt[:, 2])
+    d = dimg.reshape(-1, 2)
+    for q in range(2):
+        assert rel_err(d[:, q], odimg[:, 0, q], 1e-5).max() < 1e-7
+
+
+def test_host_build_multi_frequency(dump128):
+    W = 12
+    freqs = [8.6e10, 2.3e11, 6.9e11]
+    img, _, _ = _render(dump128, W, [30.0], freqs)
+    P = O.params(dump128.grid, width=W, height=W, inclination=30.0, M_unit=M_UNIT,
+                 R_high=R_HIGH)
+    oimg = O.render(P, dump128.prims, freqs)
+    for f in range(3):
+        assert rel_err(img[:, f], oimg[:, f]).max() < 1e-8

@@ -1,0 +1,491 @@
+"""Pins for the FP64 CPU oracle (oracle/raptor_oracle.c).
+
+The reference ships no tests, fixtures or golden vectors, and its path cannot
+be built here (it needs GSL headers the image lacks), so every oracle function
+is pinned by something that does not come from the oracle: symbolic or
+closed-form values (tests/golden, made by tests/golden/make_golden.py),
+conservation laws, textbook limits and brute-force sums.  Each test names the
+mistake it would catch.
+"""
+import json
+import math
+import os
+import warnings
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# constants.h:16-28 of the reference
+E = 4.8032068e-10
+ME = 9.1093826e-28
+MP = 1.67262171e-24
+KB = 1.3806505e-16
+C = 2.99792458e10
+H = 6.6260693e-27
+GG = 6.6742e-8
+
+
+def _grid(a=0.9375, h=0.3, n1=64, n2=64):
+    from raptor_amd.torus import make_grid
+    return make_grid(n1, n2, 1, a=a, hslope=h)
+
+
+def _P(a=0.9375, h=0.3, incl=60.0, **kw):
+    return O.params(_grid(a, h), width=kw.pop("width", 16), height=kw.pop("height", 16),
+                    inclination=incl, M_unit=kw.pop("M_unit", 1e23),
+                    R_high=kw.pop("R_high", 10.0), **kw)
+
+
+# ---------------------------------------------------------------- metric
+
+
+@pytest.fixture(scope="module")
+def kerr_gold():
+    return json.load(open(os.path.join(GOLD, "kerr_mks.json")))
+
+
+def test_metric_matches_kerr_schild_pullback(kerr_gold):
+    """g_dd and g^uu against the Cartesian Kerr-Schild form pulled back to MKS
+    (catches a wrong Jacobian factor, sign of a, or a dropped term)."""
+    for case in kerr_gold["cases"]:
+        P = _P(case["a"], case["hslope"])
+        for p, gd, gu in zip(kerr_gold["points"], case["gdd"], case["guu"]):
+            X = np.array(p)
+            gdd = O.metric_dd(P, X)
+            guu = O.metric_uu(P, X)
+            np.testing.assert_allclose(gdd, np.reshape(gd, (4, 4)), rtol=1e-12,
+                                       atol=1e-12 * np.abs(gd).max())
+            np.testing.assert_allclose(guu, np.reshape(gu, (4, 4)), rtol=1e-11,
+                                       atol=1e-12 * np.abs(gu).max())
+
+
+def test_metric_inverse_identity():
+    rng = np.random.default_rng(0)
+    P = _P()
+    for _ in range(50):
+        X = np.array([0, rng.uniform(np.log(1.4), np.log(1e4)), rng.uniform(0.01, 0.99), 1.0])
+        np.testing.assert_allclose(O.metric_dd(P, X) @ O.metric_uu(P, X), np.eye(4),
+                                   atol=1e-11)
+
+
+def test_connection_matches_exact_derivatives(kerr_gold):
+    """Gamma^m_ab against 1/2 g^md (d g) from exact symbolic derivatives
+    (catches any wrong entry, sign or symmetric copy)."""
+    for case in kerr_gold["cases"]:
+        P = _P(case["a"], case["hslope"])
+        for p, gam in zip(kerr_gold["points"], case["gamma"]):
+            G = O.connection(P, np.array(p))
+            ref = np.reshape(gam, (4, 4, 4))
+            scale = np.abs(ref).max()
+            np.testing.assert_allclose(G, ref, rtol=1e-9, atol=1e-11 * scale)
+
+
+def test_connection_symmetric_lower_indices():
+    P = _P()
+    G = O.connection(P, np.array([0, 1.3, 0.37, 0.0]))
+    np.testing.assert_array_equal(G, np.transpose(G, (0, 2, 1)))
+
+
+# ---------------------------------------------------------------- camera
+
+
+def _lower(P, y):
+    return O.metric_dd(P, y[:4]) @ y[4:]
+
+
+@pytest.mark.parametrize("incl", [10.0, 60.0, 85.0, 120.0])
+def test_camera_constants_of_motion(incl):
+    """At the camera k_t = -1 (E = 1), k_phi = -alpha sin i, k_theta = beta and
+    g(k,k) = 0 (Cunningham & Bardeen 1973; catches swapped alpha/beta or a
+    wrong BL->KS transform, which would change k_t or k_phi)."""
+    P = _P(incl=incl)
+    si = math.sin(math.radians(incl))
+    h = P.hslope
+    for alpha, beta in [(3.0, 4.0), (-7.5, 1.25), (0.3, -11.0), (15.0, 0.0)]:
+        y = O.init_photon(P, alpha, beta)
+        kd = _lower(P, y)
+        thp = math.pi * (1 + (1 - h) * math.cos(2 * math.pi * y[2]))
+        assert kd[0] == pytest.approx(-1.0, rel=1e-10)
+        assert kd[3] == pytest.approx(-alpha * si, rel=1e-9, abs=1e-9)
+        assert kd[2] / thp == pytest.approx(beta, rel=1e-8, abs=1e-8)
+        knorm = y[4:] @ O.metric_dd(P, y[:4]) @ y[4:]
+        assert abs(knorm) < 1e-10 * y[4] ** 2
+        assert y[5] > 0  # travelling inwards for negative d lambda
+
+
+def test_camera_theta_inversion():
+    """theta(X2) = theta_cam to rounding (newtonraphson.c)."""
+    lib = O.lib()
+    for h in (0.3, 0.7, 1.0):
+        for th in np.linspace(0.01, math.pi - 0.01, 25):
+            x = lib.or_theta_to_x2(th, h)
+            assert math.pi * x + 0.5 * (1 - h) * math.sin(2 * math.pi * x) == pytest.approx(
+                th, abs=2e-15)
+
+
+# ---------------------------------------------------------------- geodesics
+
+
+def _carter(P, y):
+    """Carter constant from the covariant momentum in KS/MKS."""
+    h = P.hslope
+    kd = _lower(P, y)
+    x2 = y[2]
+    th = math.pi * x2 + 0.5 * (1 - h) * math.sin(2 * math.pi * x2)
+    thp = math.pi * (1 + (1 - h) * math.cos(2 * math.pi * x2))
+    kth = kd[2] / thp
+    a = P.a
+    return kth ** 2 + math.cos(th) ** 2 * (-a * a * kd[0] ** 2 + kd[3] ** 2 / math.sin(th) ** 2)
+
+
+def _drift(P, alpha, beta):
+    path = O.ray_path(P, alpha, beta)
+    E0, L0, Q0 = _lower(P, path[0])[0], _lower(P, path[0])[3], _carter(P, path[0])
+    kd = np.array([_lower(P, y) for y in path])
+    Q = np.array([_carter(P, y) for y in path])
+    return (np.abs(kd[:, 0] - E0).max(), np.abs(kd[:, 3] - L0).max(),
+            np.abs(Q - Q0).max() / abs(Q0), len(path))
+
+
+def test_conserved_quantities_and_second_order_convergence():
+    """E = -k_t, L = k_phi and Carter's Q are constants of Kerr geodesics; the
+    RK2 midpoint error must fall ~4x when the step scale halves (an Euler or
+    misweighted step would fall only ~2x)."""
+    alpha, beta = 4.5, 3.0
+    P1 = _P(stepsize=0.03)
+    P2 = _P(stepsize=0.015)
+    e1, l1, q1, n1 = _drift(P1, alpha, beta)
+    e2, l2, q2, n2 = _drift(P2, alpha, beta)
+    L0 = 4.5 * math.sin(math.radians(60.0))
+    assert e1 < 1e-2 and l1 < 0.03 * L0 and q1 < 0.05
+    assert n2 > 1.8 * n1
+    for d1, d2 in ((e1, e2), (l1, l2), (q1, q2)):
+        ratio = d1 / d2
+        assert 3.5 < ratio < 4.5, (d1, d2)
+
+
+def test_null_condition_along_ray():
+    P = _P()
+    path = O.ray_path(P, 2.0, -6.0)
+    for y in path[:: max(1, len(path) // 40)]:
+        g = O.metric_dd(P, y[:4])
+        k = y[4:]
+        scale = np.abs(np.outer(k, k) * g).sum()
+        assert abs(k @ g @ k) < 2e-3 * scale
+
+
+def _captured(P, alpha, beta):
+    path = O.ray_path(P, alpha, beta)
+    return math.exp(path[-1][1]) < 2.0 * (1 + math.sqrt(1 - P.a ** 2))
+
+
+def test_schwarzschild_capture_threshold():
+    """a = 0: rays with impact parameter below sqrt(27) are captured."""
+    P = _P(a=0.0, h=1.0)
+    bc = math.sqrt(27.0)
+    for ang in np.linspace(0, 2 * math.pi, 7)[:-1]:
+        for b, cap in ((0.96 * bc, True), (1.04 * bc, False)):
+            assert _captured(P, b * math.cos(ang), b * math.sin(ang)) == cap, (b, ang)
+
+
+def test_kerr_shadow_boundary():
+    """a = 0.9375, i = 60 deg: points 5% inside / outside Bardeen's critical
+    curve are captured / escape (catches a wrong spin sign or frame drag)."""
+    a, i = 0.9375, math.radians(60.0)
+    P = _P(a=a, incl=60.0)
+    rp1 = 2 * (1 + math.cos(2 / 3 * math.acos(-a)))
+    rp2 = 2 * (1 + math.cos(2 / 3 * math.acos(a)))
+    n_checked = 0
+    for rr in np.linspace(rp1 + 1e-3, rp2 - 1e-3, 9):
+        xi = (rr * rr * (3 - rr) - a * a * (rr + 1)) / (a * (rr - 1))
+        eta = rr ** 3 * (4 * a * a - rr * (rr - 3) ** 2) / (a * a * (rr - 1) ** 2)
+        bsq = eta + a * a * math.cos(i) ** 2 - xi * xi / math.tan(i) ** 2
+        if bsq <= 0:
+            continue
+        alpha, beta = -xi / math.sin(i), math.sqrt(bsq)
+        assert _captured(P, 0.95 * alpha, 0.95 * beta)
+        assert not _captured(P, 1.05 * alpha, 1.05 * beta)
+        n_checked += 1
+    assert n_checked >= 5
+
+
+def test_stepsize_formula_limits():
+    """Dolence-style step: with only k^1 nonzero, dl = -eps/|k^1|; at the pole
+    (X2 = 0) the step collapses to the 1e-12 floor."""
+    P = _P()
+    assert O.stepsize(P, [0, 1, 0.5, 0], [0, 2.0, 0, 0]) == pytest.approx(-0.03 / 2.0,
+                                                                          rel=1e-12)
+    assert O.stepsize(P, [0, 1, 0.0, 0], [0, 2.0, 1.0, 0]) == -1e-12
+
+
+# ---------------------------------------------------------------- fluid
+
+
+def _linear_prims(grid, coef):
+    i = np.arange(grid.n1)[:, None, None]
+    j = np.arange(grid.n2)[None, :, None]
+    p = np.zeros((8, grid.n1, grid.n2, 1))
+    for v in range(8):
+        p[v] = coef[v][0] + coef[v][1] * i + coef[v][2] * j
+    return p
+
+
+def test_bilinear_exact_on_linear_fields_including_edge_cells():
+    """Interpolation is exact on fields linear in the cell index, including
+    the linear extrapolation over the first half cell (C truncation in Xtoij)
+    and the constant clamp over the last half cell (grmhd.c:46-71)."""
+    g = _grid(n1=16, n2=12)
+    P = O.params(g, width=4, height=4, inclination=60.0, M_unit=1e23, R_high=10.0)
+    coef = [(1.0, 0.01, 0.02), (0.1, 0.001, 0.003), (0, 0, 0), (0, 0, 0), (0, 0, 0),
+            (0, 0, 0), (0, 0, 0), (0, 0, 0)]
+    prims = _linear_prims(g, coef)
+    for fi, fj in ((3.3, 4.7), (0.2, 5.0), (7.0, 0.3), (0.1, 0.1), (15.7, 6.0), (8.0, 11.8)):
+        X = np.array([0, g.startx1 + fi * g.dx1, g.startx2 + fj * g.dx2, 0])
+        ti, tj = fi - 0.5, fj - 0.5          # continuous cell-centre index
+        ti = min(ti, g.n1 - 1.0)             # clamp past the last centre
+        tj = min(tj, g.n2 - 1.0)
+        want_rho = coef[0][0] + coef[0][1] * ti + coef[0][2] * tj
+        out = O.fluid_at(P, prims, X)
+        assert out["ne"] / O.units(P)["Ne_unit"] == pytest.approx(want_rho, rel=1e-13)
+
+
+def test_fluid_frame_invariants(dump128):
+    """u.u = -1, u.b = 0, b.b >= 0 for sampled torus points."""
+    P = O.params(dump128.grid, width=4, height=4, inclination=60.0, M_unit=1e23, R_high=10.0)
+    g = dump128.grid
+    rng = np.random.default_rng(3)
+    for _ in range(100):
+        X = np.array([0, rng.uniform(g.startx1, g.stopx1), rng.uniform(0.02, 0.98), 0])
+        f = O.fluid_at(P, dump128.prims, X)
+        gd = O.metric_dd(P, X)
+        u, b = f["ucon"], f["bcon"]
+        assert u @ gd @ u == pytest.approx(-1.0, abs=1e-10)
+        assert abs(u @ gd @ b) < 1e-10 * (1 + abs(b).max())
+        assert b @ gd @ b >= -1e-18
+
+
+def _uniform_prims(grid, rho, uu, B1):
+    p = np.zeros((8, grid.n1, grid.n2, 1))
+    p[0], p[1], p[5] = rho, uu, B1
+    return p
+
+
+def test_electron_temperature_r_beta_limits():
+    """R-beta model (Moscibrodzka et al. 2016): for beta >> 1 the ratio is
+    R_high, for beta << 1 it is R_low; Theta_e = u/rho (gam2-1) m_p/m_e/(1+R)
+    with gam2 = (1 + 2/3 (R+1)/(R+2) + 4/3)/2 (catches swapped R_high/R_low)."""
+    g = _grid(n1=8, n2=8)
+    X = np.array([0, g.startx1 + 3.3 * g.dx1, 0.41, 0])
+    for Rh, Rl in ((40.0, 1.0), (3.0, 7.0)):
+        P = O.params(g, width=4, height=4, inclination=60.0, M_unit=1e23, R_high=Rh, R_low=Rl)
+        for B1, R in ((0.0, Rh), (1e3, Rl)):
+            f = O.fluid_at(P, _uniform_prims(g, 1.0, 0.05, B1), X)
+            gam2 = 0.5 * ((1 + 2.0 / 3.0 * (R + 1) / (R + 2)) + 4.0 / 3.0)
+            want = 0.05 * (gam2 - 1) * (MP / ME) / (1 + R)
+            assert f["thetae"] == pytest.approx(want, rel=1e-6)
+
+
+def test_sigma_cut():
+    """b^2/rho > 1 makes the hook return 0 (raptor_harm_model.c:285-290)."""
+    g = _grid(n1=8, n2=8)
+    P = O.params(g, width=4, height=4, inclination=60.0, M_unit=1e23, R_high=10.0)
+    X = np.array([0, g.startx1 + 3.3 * g.dx1, 0.41, 0])
+    assert O.fluid_at(P, _uniform_prims(g, 1.0, 0.05, 1e-3), X)["ok"]
+    assert not O.fluid_at(P, _uniform_prims(g, 1.0, 0.05, 10.0), X)["ok"]
+
+
+def test_units_closed_form():
+    P = _P(M_unit=3e21)
+    u = O.units(P)
+    L = GG * 8.9505e39 / C ** 2
+    assert u["L_unit"] == pytest.approx(L, rel=1e-14)
+    assert u["RHO_unit"] * L ** 3 == pytest.approx(3e21, rel=1e-13)
+    assert u["B_unit"] ** 2 / (4 * math.pi * C ** 2) == pytest.approx(u["RHO_unit"], rel=1e-13)
+    assert u["Ne_unit"] * (MP + ME) == pytest.approx(u["RHO_unit"], rel=1e-13)
+    assert u["cutoff_inner"] == pytest.approx(1.1 * (1 + math.sqrt(1 - 0.9375 ** 2)), rel=1e-14)
+    jf = O.lib().or_jansky_factor(P)
+    assert jf == pytest.approx(1e23 * (40 * L / 2.6228263e22 / 16) ** 2, rel=1e-13)
+
+
+# ---------------------------------------------------------------- emission
+
+
+def test_thsynch_fit_against_exact_integral():
+    """The emissivity against the Maxwell-Juettner integral of the single
+    electron synchrotron spectrum (tests/golden/thsynch.json); the fit is good
+    to ~2% at Theta_e >= 30 (a dropped sqrt(2), pi or 3 is off by >= 40%)."""
+    rows = json.load(open(os.path.join(GOLD, "thsynch.json")))["rows"]
+    lib = O.lib()
+    for r in rows:
+        Th, X, sth = r["Theta"], r["X"], r["sin_theta"]
+        B, ne = 10.0, 1e5
+        nuc = E * B / (2 * math.pi * ME * C)
+        nu = X * (2.0 / 9.0) * nuc * Th ** 2 * sth
+        j = lib.or_j_thsynch(B, math.sqrt(1 - sth * sth), Th, nu, ne)
+        exact = r["j_over_ne_e2nuc_c"] * ne * (E * E / C) * nuc
+        # the reference's K2(1/Theta) ~ 2 Theta^2 (j_nu.c:146); remove that approximation
+        exact_k2approx = exact * r["K2_exact"] / (2 * Th * Th)
+        assert j / exact_k2approx == pytest.approx(1.0, abs=0.03), r
+
+
+def test_thsynch_low_frequency_slope_and_gates():
+    lib = O.lib()
+    B, Th, ne = 10.0, 20.0, 1e5
+    nuc = E * B / (2 * math.pi * ME * C)
+    nus = (2.0 / 9.0) * nuc * Th ** 2 * 0.8
+    j1 = lib.or_j_thsynch(B, 0.6, Th, 1e-9 * nus, ne)
+    j2 = lib.or_j_thsynch(B, 0.6, Th, 2e-9 * nus, ne)
+    assert math.log(j2 / j1) / math.log(2) == pytest.approx(1.0 / 3.0, abs=0.01)
+    assert lib.or_j_thsynch(B, 0.6, 0.9e-4, 1e11, ne) == 0.0   # Theta_e gate
+    assert lib.or_j_thsynch(B, 1.0, Th, 1e11, ne) == 0.0       # sin(pitch) gate
+    assert lib.or_j_thsynch(B, 0.6, Th, 1.0, ne) == 0.0        # nu_p floor gate
+    assert lib.or_j_thsynch(2 * B, 0.6, Th, 1e11, 2 * ne) > 0
+
+
+def test_kirchhoff_and_planck_limits():
+    """alpha = j / B_nu with B_nu = 2 h nu^3/c^2 / (exp(h nu/k T) - 1): the
+    reference's Planck function omits h and e2_ch/e2_c restores it."""
+    lib = O.lib()
+    assert 7.695589048e-30 == pytest.approx(E * E / C, rel=1e-7)
+    assert 1.161410e-03 == pytest.approx(E * E / (C * H), rel=1e-5)
+    for nu, Th in ((1e11, 10.0), (3e12, 0.5), (1e9, 3e-3)):
+        T = Th * ME * C * C / KB
+        x = H * nu / (KB * T)
+        Bnu = 2 * H * nu ** 3 / C ** 2 / math.expm1(x)
+        # exp(x) - 1 as in j_nu.c:190 loses ~1e-16/x relative (expm1 does not)
+        assert H * lib.or_planck(nu, Th) == pytest.approx(Bnu, rel=max(1e-9, 4e-16 / x))
+        assert lib.or_a_th(1e-20, nu, Th) * Bnu == pytest.approx(1e-20, rel=2e-5)
+    # Rayleigh-Jeans limit (x = 8e-9: correction x/2 plus cancellation 1e-16/x)
+    nu, Th = 1e12, 1.0
+    T = Th * ME * C * C / KB
+    assert H * lib.or_planck(nu, Th) == pytest.approx(2 * nu * nu * KB * T / C ** 2, rel=2e-7)
+
+
+# ---------------------------------------------------------------- transfer
+
+
+def _series(js, als, nu_p, dl):
+    lib = O.lib()
+    import ctypes
+    tau = ctypes.c_double(0.0)
+    I = ctypes.c_double(0.0)
+    dt = ctypes.c_double(0.0)
+    di = ctypes.c_double(0.0)
+    for j, a in zip(js, als):
+        lib.or_transfer_step(ctypes.byref(tau), ctypes.byref(I), j, a, nu_p, dl,
+                             ctypes.byref(dt), ctypes.byref(di))
+    return tau.value, I.value
+
+
+def test_transfer_slab_closed_form():
+    """Constant j, alpha over n end-point samples: tau_k = k d, I = (j/nu^2) dl
+    sum_k exp(-k d) -- a geometric series (radiative_transfer.c:88-96)."""
+    j, a, nu, dl, n = 2e-5, 3e-14, 1e11, 0.05, 200
+    d = nu * a * dl
+    tau, I = _series([j] * n, [a] * n, nu, dl)
+    q = math.exp(-d)
+    want = (j / nu ** 2) * dl * q * (1 - q ** n) / (1 - q)
+    assert tau == pytest.approx(n * d, rel=1e-12)
+    assert I == pytest.approx(want, rel=1e-12)
+    # optically thin limit: plain sum of j dl / nu^2
+    tau0, I0 = _series([j] * n, [0.0] * n, nu, dl)
+    assert tau0 == 0 and I0 == pytest.approx(n * j * dl / nu ** 2, rel=1e-12)
+
+
+def test_transfer_tau_gate_and_nan_poisoning():
+    """Nothing is added once tau >= ln 1000; a NaN tau (B_nu underflow with
+    j = 0) closes the gate for the rest of the ray."""
+    j, nu, dl = 1e-5, 1e11, 1.0
+    d = 0.5                                 # tau per sample
+    a = d / (nu * dl)
+    n = 30
+    tau, I = _series([j] * n, [a] * n, nu, dl)
+    kmax = max(k for k in range(1, n + 1) if k * d < math.log(1000.0))
+    want = sum((j / nu ** 2) * math.exp(-k * d) * dl for k in range(1, kmax + 1))
+    assert I == pytest.approx(want, rel=1e-12)
+    tau, I = _series([j, 0.0, j, j], [0.0, float("nan"), 0.0, 0.0], nu, dl)
+    assert math.isnan(tau) and I == pytest.approx(j / nu ** 2 * dl, rel=1e-12)
+
+
+# ---------------------------------------------------------------- tangents
+
+
+def test_tangent_pass_against_central_differences(dump128):
+    """d image / d ln(M_unit) and d image / d R_high from the tangent pass
+    against central differences.  Pixels whose image jumps between p-h and
+    p+h (a sample crossing the tau < ln 1000 or Theta_e > 1e-4 gate) are found
+    by comparing steps h and h/2 -- a jump makes the difference quotient grow
+    like 1/h -- and are excluded; they must be few."""
+    kw = dict(width=20, height=20, inclination=60.0, R_low=1.0)
+    M0, R0 = 1e23, 10.0
+    _, dimg = O.render(O.params(dump128.grid, M_unit=M0, R_high=R0, **kw), dump128.prims,
+                       [2.3e11], grad=True)
+
+    def fd(q, h):
+        if q == 0:
+            pa, pb, st = (M0 * math.exp(h), R0), (M0 * math.exp(-h), R0), 2 * h
+        else:
+            pa, pb, st = (M0, R0 * (1 + h)), (M0, R0 * (1 - h)), 2 * h * R0
+        ip = O.render(O.params(dump128.grid, M_unit=pa[0], R_high=pa[1], **kw),
+                      dump128.prims, [2.3e11])[:, 0]
+        im = O.render(O.params(dump128.grid, M_unit=pb[0], R_high=pb[1], **kw),
+                      dump128.prims, [2.3e11])[:, 0]
+        return (ip - im) / st
+
+    for q in range(2):
+        an = dimg[:, 0, q]
+        f1, f2 = fd(q, 1e-3), fd(q, 5e-4)
+        den = np.maximum(np.abs(an), 1e-3 * np.abs(an).max())
+        smooth = np.abs(f1 - f2) / den < 1e-3
+        assert smooth.mean() > 0.9, q
+        err = np.abs(f2 - an) / den
+        assert err[smooth].max() < 2e-3, (q, err[smooth].max())
+        assert f2[smooth].sum() / an[smooth].sum() == pytest.approx(1.0, abs=1e-4)
+
+
+# ---------------------------------------------------------------- data
+
+
+def test_torus_generator_properties(dump128):
+    from raptor_amd import torus
+    g = dump128.grid
+    p = dump128.prims[..., 0]
+    ex = dump128.extras
+    # density maximum near r_max = 12 on the equator
+    i, j = np.unravel_index(np.argmax(p[0]), p[0].shape)
+    assert abs(ex["r"][i, j] - 12.0) < 1.5 and abs(ex["th"][i, j] - math.pi / 2) < 0.2
+    # u.u = -1 of the stored 4-velocities
+    gd, _, _ = torus._mks_metric(ex["r"], ex["th"], ex["x2"], g.a, g.hslope)
+    uu = np.einsum("i...,...ij,j...->...", ex["ucon"], gd, ex["ucon"])
+    assert np.abs(uu + 1).max() < 1e-6
+    # plasma beta normalisation (before noise) is within the noise of 100
+    bsq = np.einsum("i...,i...->...", ex["bcon"], ex["bcov"])
+    beta_act = (g.gam - 1) * p[1].max() / (0.5 * bsq.max())
+    assert 80 < beta_act < 125
+    # every primitive is float32-representable
+    assert np.array_equal(dump128.prims.astype(np.float32).astype(np.float64), dump128.prims)
+
+
+def test_harm_ascii_and_binary_round_trip(tmp_path):
+    from raptor_amd import harm, torus
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        d = torus.fishbone_moncrief(24, 16)
+    f = tmp_path / "dump.txt"
+    harm.write_harm_ascii(str(f), d)
+    r = harm.read_harm_ascii(str(f))
+    assert np.array_equal(r.prims, d.prims)
+    assert (r.grid.n1, r.grid.n2, r.grid.a, r.grid.hslope) == (24, 16, d.grid.a, d.grid.hslope)
+    assert r.grid.startx1 == d.grid.startx1 and r.grid.dx1 == d.grid.dx1
+    line = open(f).readline().split()
+    assert len(line) == 26
+    body = open(f).read().splitlines()[1].split()
+    assert len(body) == 34
+    fb = tmp_path / "dump.bin"
+    harm.write_prims_bin(str(fb), d)
+    rb = harm.read_prims_bin(str(fb))
+    assert np.array_equal(rb.prims, d.prims) and rb.grid == d.grid
