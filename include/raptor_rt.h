@@ -151,8 +151,9 @@ int rt_memcpy_d2h(rt_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
 /* ---- the model hook on the device --------------------------------------- */
 
 /* get_fluid_params (raptor_harm_model.c:190-320) evaluated by the device
- * sampler at n points X[n][4] (MKS).  out[n][12]: Ne (cm^-3), Thetae, B (G),
- * Ucon[4], Bcon[4], and the hook's return value (1, or 0 for b^2/rho > 1). */
+ * sampler at n points X[n][4] (MKS).  out[n][14]: Ne (cm^-3), Thetae, B (G),
+ * Ucon[4], Bcon[4], the hook's return value (1, or 0 for b^2/rho > 1; then Ne,
+ * Thetae and B are 1e-20 as in the reference), beta and Bern. */
 int rt_fluid_params(rt_ctx *ctx, const double *h_X, int n, double *h_out);
 
 /* Device evaluation of the path's building blocks at n points, for checking:

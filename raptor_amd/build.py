@@ -49,7 +49,7 @@ def build_driver():
     if not srcs:
         return None
     inc = os.path.join(ROOT, "include")
-    _run(["gcc", "-O2", "-std=c99", "-Wall", "-Wextra", "-Wno-unused-parameter", "-I", inc,
+    _run(["gcc", "-O2", "-std=gnu99", "-Wall", "-Wextra", "-Wno-unused-parameter", "-I", inc,
           "-o", DRIVER, *srcs, "-L", PKG, "-lraptor_rt", "-Wl,-rpath,$ORIGIN/..", "-lm"])
     return DRIVER
 

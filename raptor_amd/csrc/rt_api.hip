@@ -174,9 +174,9 @@ __global__ void probe_fluid(const Params P, const float4 *prims, const double *i
     y[4] = 1.;
     Point pt = make_point(y[1], y[2], P.hslope);
     Fluid F;
-    double u[4], b[4];
-    bool ok = fluid_sample<DIM3>(P, prims, pt, y, F, u, b);
-    double *o = out + 12 * t;
+    double u[4], b[4], bb[2];
+    bool ok = fluid_sample<DIM3>(P, prims, pt, y, F, u, b, bb);
+    double *o = out + 14 * t;
     o[0] = ok ? F.ne : kSmall;
     o[1] = ok ? F.thetae : kSmall;
     o[2] = ok ? F.B : kSmall;
@@ -185,6 +185,8 @@ __global__ void probe_fluid(const Params P, const float4 *prims, const double *i
         o[7 + i] = b[i];
     }
     o[11] = ok ? 1. : 0.;
+    o[12] = bb[0];
+    o[13] = bb[1];
 }
 
 // cell-interleaved staging: 8 primitives of one cell in two float4
@@ -634,7 +636,7 @@ int rt_fluid_params(rt_ctx *c, const double *h_X, int n, double *h_out)
         return rc;
     double *d_in = nullptr, *d_out = nullptr;
     HIPCHK(c, hipMalloc((void **)&d_in, (size_t)n * 4 * sizeof(double)));
-    hipError_t e = hipMalloc((void **)&d_out, (size_t)n * 12 * sizeof(double));
+    hipError_t e = hipMalloc((void **)&d_out, (size_t)n * 14 * sizeof(double));
     if (e == hipSuccess)
         e = hipMemcpyAsync(d_in, h_X, (size_t)n * 4 * sizeof(double), hipMemcpyHostToDevice,
                            c->stream);
@@ -649,7 +651,7 @@ int rt_fluid_params(rt_ctx *c, const double *h_X, int n, double *h_out)
         e = hipGetLastError();
     }
     if (e == hipSuccess)
-        e = hipMemcpyAsync(h_out, d_out, (size_t)n * 12 * sizeof(double), hipMemcpyDeviceToHost,
+        e = hipMemcpyAsync(h_out, d_out, (size_t)n * 14 * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream);
     if (e == hipSuccess)
         e = hipStreamSynchronize(c->stream);

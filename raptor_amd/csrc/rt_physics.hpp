@@ -369,7 +369,8 @@ template <bool DIM3>
 __host__ __device__ inline bool fluid_sample(const Params &P, const float4 *__restrict__ prims,
                                              const Point &pt, const double y[8], Fluid &F,
                                              double *ucon_out = nullptr,
-                                             double *bcon_out = nullptr)
+                                             double *bcon_out = nullptr,
+                                             double *beta_bern_out = nullptr)
 {
     double v[8];
     sample_prims<DIM3>(P, prims, y, v);
@@ -418,6 +419,11 @@ __host__ __device__ inline bool fluid_sample(const Params &P, const float4 *__re
             ucon_out[q] = u[q];
             bcon_out[q] = b[q];
         }
+    }
+    if (beta_bern_out) {
+        // raptor_harm_model.c:263, 281 (Bernoulli parameter with gamma = 4/3)
+        beta_bern_out[0] = beta;
+        beta_bern_out[1] = -(1. + uu / rho * (4. / 3.)) * ud[0];
     }
     if (Bsq / rho > 1.0)
         return false;

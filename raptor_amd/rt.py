@@ -209,7 +209,7 @@ class Tracer:
     # -- device model hook and probes -------------------------------------------
     def fluid_params(self, X):
         X = np.ascontiguousarray(np.atleast_2d(X), float)
-        out = np.empty((X.shape[0], 12))
+        out = np.empty((X.shape[0], 14))
         self._check(self.lib.rt_fluid_params(self.ctx, _d(X), X.shape[0], _d(out)))
         return out
 

@@ -1,0 +1,133 @@
+"""The RAPTOR command-line driver (raptor_amd/host/raptor): CLI, model.in,
+dump readers and the output file formats of core.c:154-200 /
+utilities.c:47-64."""
+import os
+import subprocess
+import warnings
+
+import numpy as np
+import pytest
+
+from conftest import M_UNIT, R_HIGH
+from raptor_amd import build, harm, torus
+
+DRIVER = build.DRIVER
+
+MODEL_IN = """MBH         (g)      8.9505e39
+M_UNIT      (g)      4.e20
+ABSORPTION  (-)      1
+
+IMG_WIDTH   (pixels) {W}
+IMG_HEIGHT  (pixels) {H}
+CAM_SIZE_X  (Rg)     40
+CAM_SIZE_Y  (Rg)     40
+FREQS_PER_DEC (-)    1
+FREQ_MIN    (Hz)     2.3e11
+FREQ_MAX    (Hz)     2.3e11
+STEPSIZE    (-)      0.03
+"""
+
+
+@pytest.fixture(scope="module")
+def driver():
+    if not os.path.exists(DRIVER):
+        pytest.fail("driver not built; run `python -m raptor_amd.build`")
+    return DRIVER
+
+
+def _run(driver, args, cwd):
+    return subprocess.run([driver, *args], cwd=cwd, capture_output=True, text=True, timeout=300)
+
+
+def test_usage_and_argument_errors(driver, tmp_path):
+    r = _run(driver, [], tmp_path)
+    assert r.returncode == 2 and "usage" in r.stderr
+    (tmp_path / "model.in").write_text(MODEL_IN.format(W=8, H=8))
+    r = _run(driver, ["model.in", "d", "1e23", "60", "10", "1", "0", "--bogus", "1"], tmp_path)
+    assert r.returncode == 2 and "unknown flag" in r.stderr
+    r = _run(driver, ["model.in", "d", "x", "60", "10", "1", "0"], tmp_path)
+    assert r.returncode == 2 and "must be numbers" in r.stderr
+    r = _run(driver, ["model.in", "d", "1e23", "60", "10", "1", "0", "--freqs", "1e11,abc"],
+             tmp_path)
+    assert r.returncode == 2 and "--freqs" in r.stderr
+
+
+def test_model_in_and_dump_errors(driver, tmp_path):
+    (tmp_path / "bad.in").write_text("MBH (g) 8e39\n")
+    r = _run(driver, ["bad.in", "d", "1e23", "60", "10", "1", "0"], tmp_path)
+    assert r.returncode == 2 and "11 records" in r.stderr
+    (tmp_path / "model.in").write_text(MODEL_IN.format(W=8, H=8))
+    r = _run(driver, ["model.in", "missing", "1e23", "60", "10", "1", "0"], tmp_path)
+    assert r.returncode == 1 and "can't open sim data file" in r.stderr
+    (tmp_path / "short").write_text("0 4 4 0.3 0 0.1 0.25 0 0 0.9 1.4 0.9 0.5 0.5 1 10 0 0 0 "
+                                    "0.001 0 0 1.3 40 0.3 0\n1 2 3\n")
+    r = _run(driver, ["model.in", "short", "1e23", "60", "10", "1", "0"], tmp_path)
+    assert r.returncode == 1 and "incomplete" in r.stderr
+
+
+def _expected_files(tmp_path, image, W, H, freqs, incl, dump_name, jf, t=0):
+    """The reference's writers (core.c:163-198, utilities.c:51-61) applied to
+    an image [W*H][nf]."""
+    out = {}
+    # sequential sum over pixels, as core.c:146-150 does
+    spec = np.array([np.cumsum(image[:, f])[-1] for f in range(image.shape[1])])
+    for f, nu in enumerate(freqs):
+        name = "img_data_%d_%e_%.02f.dat" % (t, nu, incl)
+        lines = ["%d %d %e %e %s %e %e %e %e\n" % (W, H, jf * spec[f], nu, dump_name, M_UNIT,
+                                                   incl, 1.0, R_HIGH)]
+        for i in range(W):
+            for j in range(H):
+                lines.append("%+.5e\n" % (jf * image[i + j * W, f]))
+        out[name] = "".join(lines)
+    out["spectrum_%d_%.02f.dat" % (t, incl)] = "".join(
+        "%+.15e\t%+.15e\n" % (nu, jf * spec[f]) for f, nu in enumerate(freqs))
+    return out
+
+
+@pytest.mark.gpu
+def test_driver_ascii_dump_end_to_end(driver, tmp_path):
+    """ASCII HARM dump -> output files byte-identical to the reference's
+    formats applied to the library's image."""
+    from raptor_amd import rt
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        d = torus.fishbone_moncrief(48, 48)
+    harm.write_harm_ascii(str(tmp_path / "dump.txt"), d)
+    W, H = 20, 16
+    (tmp_path / "model.in").write_text(MODEL_IN.format(W=W, H=H))
+    r = _run(driver, ["model.in", "dump.txt", "%g" % M_UNIT, "60", "%g" % R_HIGH, "1", "0"],
+             tmp_path)
+    assert r.returncode == 0, r.stderr
+    with rt.Tracer(0) as tr:
+        tr.set_params(width=W, height=H, M_unit=M_UNIT, R_high=R_HIGH)
+        tr.load_dump(d)
+        img, _ = tr.render([60.0], [2.3e11])
+        jf = tr.jansky_factor()
+    want = _expected_files(tmp_path, img[0], W, H, [2.3e11], 60.0, "dump.txt", jf)
+    for name, text in want.items():
+        got = (tmp_path / "output" / name).read_text()
+        assert got == text, name
+
+
+@pytest.mark.gpu
+def test_driver_binary_3d_dump_with_lists(driver, tmp_path):
+    from raptor_amd import rt
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        d = torus.fishbone_moncrief(32, 24, 8)
+    harm.write_prims_bin(str(tmp_path / "dump.bin"), d)
+    W = H = 12
+    (tmp_path / "model.in").write_text(MODEL_IN.format(W=W, H=H))
+    freqs = [8.6e10, 2.3e11]
+    r = _run(driver, ["model.in", "dump.bin", "%g" % M_UNIT, "60", "%g" % R_HIGH, "1", "0",
+                      "--incl", "30,60", "--freqs", "8.6e10,2.3e11"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    with rt.Tracer(0) as tr:
+        tr.set_params(width=W, height=H, M_unit=M_UNIT, R_high=R_HIGH)
+        tr.load_dump(d)
+        img, _ = tr.render([30.0, 60.0], freqs)
+        jf = tr.jansky_factor()
+    for k, inc in enumerate((30.0, 60.0)):
+        want = _expected_files(tmp_path, img[k], W, H, freqs, inc, "dump.bin", jf)
+        for name, text in want.items():
+            assert (tmp_path / "output" / name).read_text() == text, name
