@@ -34,7 +34,6 @@ struct TraceArgs {
     double *image;
     double *dimage;
     rt_ray_counters *counters;
-    double nu3[kMaxFreq];
 };
 
 template <int NF, bool GRAD, bool DIM3>
@@ -54,10 +53,10 @@ __global__ __launch_bounds__(64) void trace_kernel(const Params P, const TraceAr
             const size_t o = (size_t)ray * P.nf;
             for (int f = 0; f < NF; f++) {
                 if (f < P.nf) {
-                    A.image[o + f] = R.acc.I[f] * A.nu3[f];
+                    A.image[o + f] = (double)R.acc.I[f];
                     if (GRAD) {
-                        A.dimage[(o + f) * 2 + 0] = R.acc.dI[f][0] * A.nu3[f];
-                        A.dimage[(o + f) * 2 + 1] = R.acc.dI[f][1] * A.nu3[f];
+                        A.dimage[(o + f) * 2 + 0] = R.acc.dI[f][0];
+                        A.dimage[(o + f) * 2 + 1] = R.acc.dI[f][1];
                     }
                 }
             }
@@ -153,10 +152,10 @@ __global__ void probe_emission(const double *in, int n, double *out)
     if (t >= n)
         return;
     const double *q = in + 5 * t;
-    double j, an, X;
-    emission(q[0], q[1], q[2], q[3], q[4], j, an, X);
-    out[2 * t] = j;
-    out[2 * t + 1] = an;
+    EmisF e = emission_f32((float)q[0], (float)q[1], (float)q[2], (float)q[3], (float)q[4]);
+    double jx = (double)e.jh * exp(-(double)e.cb);
+    out[2 * t] = kE2C * jx;
+    out[2 * t + 1] = e.jh != 0.f ? jx * (double)e.kap : (e.poison ? NAN : 0.);
 }
 
 template <bool DIM3>
@@ -174,7 +173,7 @@ __global__ void probe_fluid(const Params P, const float4 *prims, const double *i
     y[4] = 1.;
     Point pt = make_point(y[1], y[2], P.hslope);
     Fluid F;
-    double u[4], b[4], bb[2];
+    float u[4], b[4], bb[2];
     bool ok = fluid_sample<DIM3>(P, prims, pt, y, F, u, b, bb);
     double *o = out + 14 * t;
     o[0] = ok ? F.ne : kSmall;
@@ -247,7 +246,7 @@ static int select_device(rt_ctx *c)
 
 // Fills the launch parameters from the context; validates sizes.
 static int make_params(rt_ctx *c, const double *incl, int ncam, const double *freqs, int nf,
-                       Params &P, double nu3[kMaxFreq])
+                       Params &P)
 {
     if (!c->have_params)
         return fail(c, RT_E_STATE, "rt_set_params has not been called");
@@ -264,7 +263,7 @@ static int make_params(rt_ctx *c, const double *incl, int ncam, const double *fr
     long long total = (long long)ncam * p.width * p.height;
     if (total > 0x7fffff00LL)
         return fail(c, RT_E_INVALID, "too many rays in one launch (%lld)", total);
-    const char *msg = fill_params(p, g, incl, ncam, freqs, nf, P, nu3);
+    const char *msg = fill_params(p, g, incl, ncam, freqs, nf, P);
     if (msg)
         return fail(c, RT_E_INVALID, "%s", msg);
     return RT_OK;
@@ -322,7 +321,7 @@ static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *fr
         return RT_E_INVALID;
     Params P;
     TraceArgs A;
-    int rc = make_params(c, incl, ncam, freqs, nf, P, A.nu3);
+    int rc = make_params(c, incl, ncam, freqs, nf, P);
     if (rc)
         return rc;
     if (!d_image || (grad && !d_dimage))
@@ -628,8 +627,7 @@ int rt_fluid_params(rt_ctx *c, const double *h_X, int n, double *h_out)
         return RT_OK;
     const double one = 2.3e11, incl = 60.;
     Params P;
-    double nu3[kMaxFreq];
-    int rc = make_params(c, &incl, 1, &one, 1, P, nu3);
+    int rc = make_params(c, &incl, 1, &one, 1, P);
     if (rc)
         return rc;
     if ((rc = select_device(c)))
@@ -703,8 +701,7 @@ int rt_probe(rt_ctx *c, int kind, const double *h_in, int n, double *h_out)
             incl.push_back(60.);
         }
         const double nu = 2.3e11;
-        double nu3[kMaxFreq];
-        int rc = make_params(c, incl.data(), (int)incl.size(), &nu, 1, P, nu3);
+        int rc = make_params(c, incl.data(), (int)incl.size(), &nu, 1, P);
         if (rc)
             return rc;
     }

@@ -81,7 +81,9 @@ struct Params {
     double R_high, R_low;
     int nf;
     double freq[kMaxFreq];
-    double dl_scale[kMaxFreq];   // Rg / nu_f
+    // FP32 copies for the transfer
+    float fnu[kMaxFreq];
+    float fNe_unit, fB_unit, fR_high, fR_low, fa, fpi_h;   // fpi_h = (1 - h) pi
     int ncam;
     Camera cam[kMaxCam];
 };
@@ -271,10 +273,12 @@ __host__ __device__ inline void init_photon(const Camera &cm, double alpha, doub
 }
 
 // Interpolated primitives at a point (grmhd.c:35-71, with the 3D extension).
+// The cell index and the weights come from the FP64 position; the weighted
+// sums are formed in FP32 from the FP32 primitives.
 // prims: 8 floats per cell, cells ordered ((i*n2 + j)*n3 + k).
 template <bool DIM3>
 __host__ __device__ inline void sample_prims(const Params &P, const float4 *__restrict__ prims,
-                                             const double y[8], double v[8])
+                                             const double y[8], float v[8])
 {
     int i = (int)((y[1] - P.startx1) / P.dx1 - 0.5);
     int j = (int)((y[2] - P.startx2) / P.dx2 - 0.5);
@@ -297,8 +301,9 @@ __host__ __device__ inline void sample_prims(const Params &P, const float4 *__re
     } else {
         d2 = (y[2] - ((j + 0.5) * P.dx2 + P.startx2)) / P.dx2;
     }
-    const double c0 = (1. - d1) * (1. - d2), c1 = (1. - d1) * d2;
-    const double c2 = d1 * (1. - d2), c3 = d1 * d2;
+    const float c0 = (float)((1. - d1) * (1. - d2)), c1 = (float)((1. - d1) * d2);
+    const float c2 = (float)(d1 * (1. - d2)), c3 = (float)(d1 * d2);
+#define RT_BI(f00, f01, f10, f11) ((f00) * c0 + (f01) * c1 + (f10) * c2 + (f11) * c3)
     if (!DIM3) {
         const size_t b00 = (size_t)(i * P.n2 + j) * 2;
         const size_t b10 = b00 + (size_t)P.n2 * 2;
@@ -306,8 +311,6 @@ __host__ __device__ inline void sample_prims(const Params &P, const float4 *__re
         float4 p01a = prims[b00 + 2], p01b = prims[b00 + 3];
         float4 p10a = prims[b10], p10b = prims[b10 + 1];
         float4 p11a = prims[b10 + 2], p11b = prims[b10 + 3];
-#define RT_BI(f00, f01, f10, f11) \
-    ((double)(f00) * c0 + (double)(f01) * c1 + (double)(f10) * c2 + (double)(f11) * c3)
         v[0] = RT_BI(p00a.x, p01a.x, p10a.x, p11a.x);
         v[1] = RT_BI(p00a.y, p01a.y, p10a.y, p11a.y);
         v[2] = RT_BI(p00a.z, p01a.z, p10a.z, p11a.z);
@@ -324,13 +327,13 @@ __host__ __device__ inline void sample_prims(const Params &P, const float4 *__re
             ph += period;
         double t = ph / P.dx3 - 0.5;
         double fk = floor(t);
-        double d3 = t - fk;
+        const float d3 = (float)(t - fk);
         int k = (int)fk;
         int k0 = (k + P.n3) % P.n3, k1 = (k + 1) % P.n3;
         const size_t n3 = P.n3, n2 = P.n2;
         size_t b00 = ((size_t)i * n2 + j) * n3, b01 = b00 + n3;
         size_t b10 = b00 + n2 * n3, b11 = b10 + n3;
-        double w[2][8];
+        float w[2][8];
         const int ks[2] = {k0, k1};
 #pragma unroll
         for (int q = 0; q < 2; q++) {
@@ -348,71 +351,110 @@ __host__ __device__ inline void sample_prims(const Params &P, const float4 *__re
             w[q][6] = RT_BI(p00b.z, p01b.z, p10b.z, p11b.z);
             w[q][7] = RT_BI(p00b.w, p01b.w, p10b.w, p11b.w);
         }
-#undef RT_BI
         for (int q = 0; q < 8; q++)
-            v[q] = (1. - d3) * w[0][q] + d3 * w[1][q];
+            v[q] = (1.f - d3) * w[0][q] + d3 * w[1][q];
     }
+#undef RT_BI
 }
 
-// The fluid state the transfer needs at one sample.
+// The fluid state the transfer needs at one sample (FP32).
 struct Fluid {
-    double ne, thetae, B;   // cgs density, dimensionless temperature, gauss
-    double mu;              // cos(pitch angle)
-    double gdop;            // nu_plasma / nu_observed = -(u . k)
-    double dthetae;         // d Thetae / d R_high
+    float ne, thetae, B;   // cgs density, dimensionless temperature, gauss
+    float mu;              // cos(pitch angle)
+    float gdop;            // nu_plasma / nu_observed = -(u . k)
+    float dthetae;         // d Thetae / d R_high
 };
 
+// Covariant MKS metric in FP32 from the point's FP64 trigonometry.
+struct MetricF {
+    float g00, g01, g03, g11, g13, g22, g33;
+    float gcon00, gcon01;  // the two inverse-metric entries the fluid needs
+};
+
+__host__ __device__ inline MetricF metric_f32(const Point &pt, const Params &P)
+{
+    MetricF g;
+    const float r = (float)pt.r, s = (float)pt.s, c = (float)pt.c;
+    const float a = P.fa;
+    const float sin2 = s * s;
+    const float rho2 = r * r + a * a * c * c;
+    const float hfac = P.fpi_h * (float)pt.c2 + 3.14159265f;
+    const float irho2 = 1.f / rho2;
+    const float z = 2.f * r * irho2;
+    g.g00 = -1.f + z;
+    g.g01 = z * r;
+    g.g03 = -a * z * sin2;
+    g.g11 = (1.f + z) * r * r;
+    g.g13 = -a * sin2 * (1.f + z) * r;
+    g.g22 = rho2 * hfac * hfac;
+    g.g33 = sin2 * (rho2 + a * a * sin2 * (1.f + z));
+    g.gcon00 = -1.f - z;
+    g.gcon01 = 2.f * irho2;
+    return g;
+}
+
+__host__ __device__ inline void lower_f(const MetricF &g, const float v[4], float o[4])
+{
+    o[0] = g.g00 * v[0] + g.g01 * v[1] + g.g03 * v[3];
+    o[1] = g.g01 * v[0] + g.g11 * v[1] + g.g13 * v[3];
+    o[2] = g.g22 * v[2];
+    o[3] = g.g03 * v[0] + g.g13 * v[1] + g.g33 * v[3];
+}
+
 // get_fluid_params (raptor_harm_model.c:190-320) + pitch_angle and the
-// plasma-frame frequency factor (radiative_transfer.c:17-47).  Returns false
-// when b^2/rho > 1 (the sample is skipped).
+// plasma-frame frequency factor (radiative_transfer.c:17-47), in FP32.
+// Returns false when b^2/rho > 1 (the sample is skipped).  The electron
+// temperature ratio is formed as R_high + (R_low - R_high)/(1 + beta^2),
+// which equals the reference's expression and stays finite when beta^2
+// overflows FP32.
 template <bool DIM3>
 __host__ __device__ inline bool fluid_sample(const Params &P, const float4 *__restrict__ prims,
                                              const Point &pt, const double y[8], Fluid &F,
-                                             double *ucon_out = nullptr,
-                                             double *bcon_out = nullptr,
-                                             double *beta_bern_out = nullptr)
+                                             float *ucon_out = nullptr,
+                                             float *bcon_out = nullptr,
+                                             float *beta_bern_out = nullptr)
 {
-    double v[8];
+    float v[8];
     sample_prims<DIM3>(P, prims, y, v);
-    const double rho = v[0], uu = v[1];
-    const double a = P.a;
-    MetricDD g = metric_dd(pt, a, P.hslope);
-    // the two entries of g^{mu nu} the velocity reconstruction uses (metric.c:221-222)
-    double r = pt.r;
-    double irho2 = 1. / (r * r + a * a * pt.c * pt.c);
-    double gcon00 = -1. - 2. * r * irho2;
-    double gcon01 = 2. * irho2;
-    const double V1 = v[2], V2 = v[3], V3 = v[4];
-    double VdotV = g.g11 * V1 * V1 + g.g13 * V1 * V3 + g.g22 * V2 * V2 + g.g13 * V3 * V1 +
-                   g.g33 * V3 * V3;
-    double Vfac = sqrt(-1. / gcon00 * (1. + fabs(VdotV)));
-    double u[4], ud[4], b[4], bd[4];
-    u[0] = -Vfac * gcon00;
-    u[1] = V1 - Vfac * gcon01;
+    const float rho = v[0], uu = v[1];
+    const MetricF g = metric_f32(pt, P);
+    const float V1 = v[2], V2 = v[3], V3 = v[4];
+    float VdotV = g.g11 * V1 * V1 + g.g13 * V1 * V3 + g.g22 * V2 * V2 + g.g13 * V3 * V1 +
+                  g.g33 * V3 * V3;
+    float Vfac = sqrtf(-1.f / g.gcon00 * (1.f + fabsf(VdotV)));
+    float u[4], ud[4], b[4], bd[4];
+    u[0] = -Vfac * g.gcon00;
+    u[1] = V1 - Vfac * g.gcon01;
     u[2] = V2;
     u[3] = V3;
-    lower(g, u, ud);
-    double UdotBp = ud[1] * v[5] + ud[2] * v[6] + ud[3] * v[7];
+    lower_f(g, u, ud);
+    float UdotBp = ud[1] * v[5] + ud[2] * v[6] + ud[3] * v[7];
+    const float iu0 = 1.f / u[0];
     b[0] = UdotBp;
-    b[1] = (v[5] + u[1] * UdotBp) / u[0];
-    b[2] = (v[6] + u[2] * UdotBp) / u[0];
-    b[3] = (v[7] + u[3] * UdotBp) / u[0];
-    lower(g, b, bd);
-    double Bsq = b[0] * bd[0] + b[1] * bd[1] + b[2] * bd[2] + b[3] * bd[3];
-    F.ne = rho * P.Ne_unit;
-    F.B = sqrt(Bsq) * P.B_unit + kSmall;
-    double beta = uu * (4. / 3. - 1.) / (0.5 * (Bsq + kSmall));
-    double b2 = beta * beta;
-    double trat = P.R_high * b2 / (1. + b2) + P.R_low / (1. + b2);
-    double gam2 = 0.5 * ((1. + 2. / 3. * (trat + 1.) / (trat + 2.)) + 4. / 3.);
-    const double mr = kProtonMass / kElectronMass;
-    F.thetae = (uu / rho) * ((gam2 - 1.) * mr / (1. + trat));
-    double dgam2 = (1. / 3.) / ((trat + 2.) * (trat + 2.));
-    double dunit = mr * (dgam2 * (1. + trat) - (gam2 - 1.)) / ((1. + trat) * (1. + trat));
-    F.dthetae = (uu / rho) * dunit * (b2 / (1. + b2));
-    if (F.thetae < 0.) {
-        F.thetae = kSmall;
-        F.dthetae = 0.;
+    b[1] = (v[5] + u[1] * UdotBp) * iu0;
+    b[2] = (v[6] + u[2] * UdotBp) * iu0;
+    b[3] = (v[7] + u[3] * UdotBp) * iu0;
+    lower_f(g, b, bd);
+    float Bsq = b[0] * bd[0] + b[1] * bd[1] + b[2] * bd[2] + b[3] * bd[3];
+    F.ne = rho * P.fNe_unit;
+    const float bnorm = sqrtf(Bsq);
+    F.B = bnorm * P.fB_unit + 1e-20f;
+    float beta = uu * (1.f / 3.f) / (0.5f * (Bsq + 1e-20f));
+    float b2 = beta * beta;
+    float w = 1.f / (1.f + b2);
+    float trat = P.fR_high + (P.fR_low - P.fR_high) * w;
+    float itr2 = 1.f / (trat + 2.f);
+    float gam2 = 0.5f * ((1.f + (2.f / 3.f) * (trat + 1.f) * itr2) + 4.f / 3.f);
+    const float mr = (float)(kProtonMass / kElectronMass);
+    const float itr1 = 1.f / (1.f + trat);
+    const float uor = uu / rho;
+    F.thetae = uor * ((gam2 - 1.f) * mr * itr1);
+    float dgam2 = (1.f / 3.f) * itr2 * itr2;
+    const float dtrat = (b2 <= 3e38f) ? b2 * w : 1.f;   // d trat / d R_high
+    F.dthetae = uor * mr * (dgam2 * (1.f + trat) - (gam2 - 1.f)) * itr1 * itr1 * dtrat;
+    if (F.thetae < 0.f) {
+        F.thetae = 1e-20f;
+        F.dthetae = 0.f;
     }
     if (ucon_out) {
         for (int q = 0; q < 4; q++) {
@@ -423,70 +465,91 @@ __host__ __device__ inline bool fluid_sample(const Params &P, const float4 *__re
     if (beta_bern_out) {
         // raptor_harm_model.c:263, 281 (Bernoulli parameter with gamma = 4/3)
         beta_bern_out[0] = beta;
-        beta_bern_out[1] = -(1. + uu / rho * (4. / 3.)) * ud[0];
+        beta_bern_out[1] = -(1.f + uor * (4.f / 3.f)) * ud[0];
     }
-    if (Bsq / rho > 1.0)
+    if (Bsq / rho > 1.f)
         return false;
     // pitch angle (cos) and Doppler factor with k lowered by the same metric
-    double kd[4];
-    lower(g, y + 4, kd);
-    double ku = kd[0] * u[0] + kd[1] * u[1] + kd[2] * u[2] + kd[3] * u[3];
-    double mu = (kd[0] * b[0] + kd[1] * b[1] + kd[2] * b[2] + kd[3] * b[3]) /
-                (fabs(ku) * F.B / P.B_unit);
-    if (fabs(mu) > 1.)
-        mu /= fabs(mu);
+    float k[4] = {(float)y[4], (float)y[5], (float)y[6], (float)y[7]};
+    float kd[4];
+    lower_f(g, k, kd);
+    float ku = kd[0] * u[0] + kd[1] * u[1] + kd[2] * u[2] + kd[3] * u[3];
+    float mu = (kd[0] * b[0] + kd[1] * b[1] + kd[2] * b[2] + kd[3] * b[3]) /
+               (fabsf(ku) * (F.B / P.fB_unit));
+    if (fabsf(mu) > 1.f)
+        mu /= fabsf(mu);
     F.mu = mu;
     F.gdop = -ku;
     return true;
 }
 
 // Thermal synchrotron emissivity (j_nu.c:130-153) and Kirchhoff absorptivity
-// (j_nu.c:188-200).  X^(1/2) and X^(1/6), X^(1/3) are sqrt and cube roots.
-__host__ __device__ inline void emission(double B, double mu, double Te, double nu_p, double ne,
-                                         double &j, double &an, double &X_out)
+// (j_nu.c:188-200) in FP32 and scaled units:
+//   jh   = j_nu / (e^2/c) without its exp(-X^(1/3)) factor; kept as
+//          n_e sqrt(2) pi / 3 * (nu_s / K2) f with nu_s / K2 = nu_c sin / 9
+//          since K2 = 2 Theta_e^2 cancels Theta_e^2 in nu_s;
+//   cb   = X^(1/3), so j_nu = (e^2/c) jh exp(-cb);
+//   kap  = alpha_nu / (jh exp(-cb)) = e2_ch (exp(x)-1) c^2 / (2 nu_p^3),
+//          x = h nu_p / (k T_e), the reference's Planck function without h.
+// jh == 0 marks the reference's zero-emissivity gates.  When j is zero and the
+// reference's FP64 exp(x) overflows (x > 709.78, only for Theta_e below
+// ~1e-11) its absorptivity is 0/0 = NaN; `poison` reports that case.
+struct EmisF {
+    float jh, cb, kap, X;
+    bool poison;
+};
+
+__host__ __device__ inline EmisF emission_f32(float B, float mu, float Te, float nu_p, float ne)
 {
-    double sth = sqrt(1. - mu * mu);
-    double nu_c = kElectronCharge * B / (2. * kPi * kElectronMass * kC);
-    double nu_s = (2. / 9.) * nu_c * Te * Te * sth;
-    double X = nu_p / nu_s;
-    X_out = X;
-    double cb = cbrt(X);
-    double sx = sqrt(X);
-    const double c1112 = 1.8877486253633868;   // 2^(11/12)
-    double ff = sx + c1112 * sqrt(cb);
-    ff = ff * ff;
-    if (nu_p == 1. || fabs(sth) < 1.e-10 || !(Te > 1e-4)) {
-        j = 0.;
-    } else {
-        double K2 = 2. * Te * Te;
-        j = ne * sqrt(2.) * kPi * nu_s / (3. * K2) * ff * exp(-cb) * kE2C;
+    EmisF e;
+    const float sth = sqrtf(1.f - mu * mu);
+    const float nu_c = (float)(kElectronCharge / (2. * kPi * kElectronMass * kC)) * B;
+    const float nu_s = (2.f / 9.f) * nu_c * Te * Te * sth;
+    const float X = nu_p / nu_s;
+    e.X = X;
+    e.poison = false;
+    if (nu_p == 1.f || fabsf(sth) < 1.e-10f || !(Te > 1e-4f)) {
+        e.jh = 0.f;
+        e.cb = 0.f;
+        e.kap = 0.f;
+        if (!(Te > 1e-4f)) {
+            double T = (double)Te * kElectronMass * kC * kC / kBoltzmann;
+            e.poison = kPlanck * (double)nu_p / (kBoltzmann * T) > 709.782712893384;
+        }
+        return e;
     }
-    double T = Te * kElectronMass * kC * kC / kBoltzmann;
-    double Bnu = 2. * nu_p * nu_p * nu_p / (kC * kC) * 1. /
-                 (exp(kPlanck * nu_p / (kBoltzmann * T)) - 1.);
-    an = j * kE2CH / Bnu / kE2C;
+    const float cb = cbrtf(X);
+    const float ff = sqrtf(X) + 1.8877486253633868f * sqrtf(cb);   // 2^(11/12) X^(1/6)
+    e.cb = cb;
+    e.jh = ne * (float)(1.4142135623730951 * kPi / 3. / 9.) * (nu_c * sth) * (ff * ff);
+    const float x = (float)(kPlanck / (kElectronMass * kC * kC)) * nu_p / Te;
+    const float cn = (float)kC / nu_p;
+    e.kap = (float)(0.5 * kE2CH) * expm1f(x) * (cn * cn) / nu_p;
+    return e;
 }
 
 // d ln j / d ln X of the fit at fixed n_e and nu_s
-__host__ __device__ inline double dlnj_dlnX(double X)
+__host__ __device__ inline float dlnj_dlnX(float X, float cb)
 {
-    const double c1112 = 1.8877486253633868;
-    double sx = sqrt(X), cb = cbrt(X), t = sqrt(cb);
-    return 2. * (0.5 * sx + c1112 / 6. * t) / (sx + c1112 * t) - cb / 3.;
+    const float c1112 = 1.8877486253633868f;
+    float sx = sqrtf(X), t = sqrtf(cb);
+    return 2.f * (0.5f * sx + c1112 / 6.f * t) / (sx + c1112 * t) - cb * (1.f / 3.f);
 }
 
-// Per-ray accumulators for NF frequencies.
+// Per-ray accumulators for NF frequencies: optical depth and I_nu nu^3
+// (FP32 with Kahan compensation), and the tangents (FP64: they difference
+// nearly equal terms, d ln j - d tau).
 template <int NF, bool GRAD>
 struct Accum {
-    double tau[NF];
-    double I[NF];
+    float tau[NF], ctau[NF];
+    float I[NF], cI[NF];
     double dtau[GRAD ? NF : 1][2];
     double dI[GRAD ? NF : 1][2];
     __host__ __device__ void clear()
     {
         for (int f = 0; f < NF; f++) {
-            tau[f] = 0.;
-            I[f] = 0.;
+            tau[f] = ctau[f] = 0.f;
+            I[f] = cI[f] = 0.f;
             if (GRAD) {
                 dtau[f][0] = dtau[f][1] = 0.;
                 dI[f][0] = dI[f][1] = 0.;
@@ -495,46 +558,59 @@ struct Accum {
     }
 };
 
+__host__ __device__ inline void kahan_add(float &s, float &c, float x)
+{
+    float yk = x - c;
+    float t = s + yk;
+    c = (t - s) - yk;
+    s = t;
+}
+
 // radiative_transfer.c:71-98 for one sample: end-point quadrature with the
-// step's |dlambda|, tau gate 0 <= tau < ln 1000.
+// step's |dlambda|, tau gate 0 <= tau < ln 1000.  With g = nu_p / nu and
+// dl = |dlambda| R_g:
+//   dtau = nu_p alpha_nu dl / nu = g alpha_nu dl
+//   dI nu^3 = (j_nu / nu_p^2) exp(-tau) (dl / nu) nu^3 = j_nu exp(-tau) dl / g^2
 template <int NF, bool GRAD>
 __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F, double dlabs,
                                                 Accum<NF, GRAD> &acc)
 {
-    const double lnth = 6.907755278982137;   // log(1000.)
+    const float lnth = 6.907755278982137f;   // log(1000.)
+    const float dl = (float)(dlabs * P.Rg);
 #pragma unroll
     for (int f = 0; f < NF; f++) {
-        double nu_p = F.gdop * P.freq[f];
-        if (nu_p <= 0.)
-            nu_p = 1.;
-        double j, an, X;
-        emission(F.B, F.mu, F.thetae, nu_p, F.ne, j, an, X);
-        double dlf = dlabs * P.dl_scale[f];
-        double dtau = nu_p * an * dlf;
-        acc.tau[f] += dtau;
-        double dI = (j / (nu_p * nu_p)) * exp(-acc.tau[f]) * dlf;
-        double dlnj0 = 0., dlnj1 = 0., dlna1 = 0.;
-        if (GRAD && j != 0.) {
-            double Dx = dlnj_dlnX(X);
-            double dlnB = 0.5 * (F.B - kSmall) / F.B;
-            double dlnT = F.dthetae / F.thetae;
-            double xx = kPlanck * nu_p / (F.thetae * kElectronMass * kC * kC);
-            double dlnBnu = xx / (-expm1(-xx));
-            dlnj0 = 1. + dlnB * (1. - Dx);
-            dlnj1 = -2. * Dx * dlnT;
+        float g = F.gdop;
+        float nu_p = g * P.fnu[f];
+        if (nu_p <= 0.f) {
+            nu_p = 1.f;
+            g = 1.f / P.fnu[f];
+        }
+        const EmisF e = emission_f32(F.B, F.mu, F.thetae, nu_p, F.ne);
+        const float jx = e.jh * expf(-e.cb);
+        float dtau = e.jh != 0.f ? g * dl * jx * e.kap : (e.poison ? NAN : 0.f);
+        kahan_add(acc.tau[f], acc.ctau[f], dtau);
+        const float tau = acc.tau[f];
+        const float dI = (float)kE2C * jx * expf(-tau) * (dl / (g * g));
+        float dlnj0 = 0.f, dlnj1 = 0.f, dlna1 = 0.f;
+        if (GRAD && e.jh != 0.f) {
+            float Dx = dlnj_dlnX(e.X, e.cb);
+            float dlnB = 0.5f * (F.B - 1e-20f) / F.B;
+            float dlnT = F.dthetae / F.thetae;
+            float x = (float)(kPlanck / (kElectronMass * kC * kC)) * nu_p / F.thetae;
+            float dlnBnu = x / (-expm1f(-x));
+            dlnj0 = 1.f + dlnB * (1.f - Dx);
+            dlnj1 = -2.f * Dx * dlnT;
             dlna1 = dlnj1 - dlnBnu * dlnT;
         }
-        if (GRAD) {
-            if (dtau != 0.) {
-                acc.dtau[f][0] += dtau * dlnj0;
-                acc.dtau[f][1] += dtau * dlna1;
-            }
+        if (GRAD && dtau != 0.f) {
+            acc.dtau[f][0] += (double)dtau * dlnj0;
+            acc.dtau[f][1] += (double)dtau * dlna1;
         }
-        if (acc.tau[f] < lnth && !(acc.tau[f] < 0.)) {
-            acc.I[f] += dI;
-            if (GRAD && dI != 0.) {
-                acc.dI[f][0] += dI * (dlnj0 - acc.dtau[f][0]);
-                acc.dI[f][1] += dI * (dlnj1 - acc.dtau[f][1]);
+        if (tau < lnth && !(tau < 0.f)) {
+            kahan_add(acc.I[f], acc.cI[f], dI);
+            if (GRAD && dI != 0.f) {
+                acc.dI[f][0] += (double)dI * (dlnj0 - acc.dtau[f][0]);
+                acc.dI[f][1] += (double)dI * (dlnj1 - acc.dtau[f][1]);
             }
         }
     }
@@ -670,7 +746,7 @@ inline Camera make_camera(double incl_deg, double a, double h)
 // utilities.c:17-21 cutoff).  Returns NULL or an error message.
 template <class RtParams, class RtGrid>
 inline const char *fill_params(const RtParams &p, const RtGrid &g, const double *incl, int ncam,
-                               const double *freqs, int nf, Params &P, double nu3[kMaxFreq])
+                               const double *freqs, int nf, Params &P)
 {
     memset(&P, 0, sizeof P);
     P.n1 = g.n1;
@@ -702,14 +778,19 @@ inline const char *fill_params(const RtParams &p, const RtGrid &g, const double 
     P.Rg = kGGrav * p.MBH / kC / kC;
     P.R_high = p.R_high;
     P.R_low = p.R_low;
+    P.fNe_unit = (float)P.Ne_unit;
+    P.fB_unit = (float)P.B_unit;
+    P.fR_high = (float)p.R_high;
+    P.fR_low = (float)p.R_low;
+    P.fa = (float)g.a;
+    P.fpi_h = (float)((1. - g.hslope) * kPi);
     P.nf = nf;
     for (int f = 0; f < kMaxFreq; f++) {
         double nu = freqs[std::min(f, nf - 1)];
         if (!(nu > 0.))
             return "frequencies must be positive";
         P.freq[f] = nu;
-        P.dl_scale[f] = P.Rg / nu;
-        nu3[f] = std::pow(nu, 3.);
+        P.fnu[f] = (float)nu;
     }
     P.ncam = ncam;
     for (int i = 0; i < ncam; i++) {

@@ -13,8 +13,8 @@
 using namespace rt;
 
 template <int NF, bool GRAD>
-static void run(const Params &P, const float4 *prims, const double *nu3, int total, double *img,
-                double *dimg, int *cnt)
+static void run(const Params &P, const float4 *prims, int total, double *img, double *dimg,
+                int *cnt)
 {
     for (int ray = 0; ray < total; ray++) {
         Ray<NF, GRAD> R;
@@ -27,10 +27,10 @@ static void run(const Params &P, const float4 *prims, const double *nu3, int tot
                 break;
         }
         for (int f = 0; f < P.nf; f++) {
-            img[(size_t)ray * P.nf + f] = R.acc.I[f] * nu3[f];
+            img[(size_t)ray * P.nf + f] = R.acc.I[f];
             if (GRAD) {
-                dimg[((size_t)ray * P.nf + f) * 2] = R.acc.dI[f][0] * nu3[f];
-                dimg[((size_t)ray * P.nf + f) * 2 + 1] = R.acc.dI[f][1] * nu3[f];
+                dimg[((size_t)ray * P.nf + f) * 2] = R.acc.dI[f][0];
+                dimg[((size_t)ray * P.nf + f) * 2 + 1] = R.acc.dI[f][1];
             }
         }
         if (cnt) {
@@ -45,8 +45,7 @@ extern "C" int hc_render(const rt_params *p, const rt_grid *g, const float *soa,
                          double *dimg, int *cnt)
 {
     Params *P = new Params;
-    double nu3[kMaxFreq];
-    const char *msg = fill_params(*p, *g, incl, ncam, freqs, nf, *P, nu3);
+    const char *msg = fill_params(*p, *g, incl, ncam, freqs, nf, *P);
     if (msg) {
         delete P;
         return -1;
@@ -60,11 +59,11 @@ extern "C" int hc_render(const rt_params *p, const rt_grid *g, const float *soa,
     }
     int total = ncam * P->npix;
     if (nf == 1) {
-        if (dimg) run<1, true>(*P, aos.data(), nu3, total, img, dimg, cnt);
-        else run<1, false>(*P, aos.data(), nu3, total, img, dimg, cnt);
+        if (dimg) run<1, true>(*P, aos.data(), total, img, dimg, cnt);
+        else run<1, false>(*P, aos.data(), total, img, dimg, cnt);
     } else {
-        if (dimg) run<16, true>(*P, aos.data(), nu3, total, img, dimg, cnt);
-        else run<16, false>(*P, aos.data(), nu3, total, img, dimg, cnt);
+        if (dimg) run<16, true>(*P, aos.data(), total, img, dimg, cnt);
+        else run<16, false>(*P, aos.data(), total, img, dimg, cnt);
     }
     delete P;
     return 0;

@@ -93,7 +93,9 @@ def test_gradients_match_oracle(tracer, dump128):
     for q in range(2):
         g, o = dimg[0, :, 0, q], odimg[:, 0, q]
         assert abs(g.sum() / o.sum() - 1) < 1e-4
-        assert rel_err(g, o, 1e-5).max() < 1e-3
+        # FP32 transfer: per-pixel gradients are sums of terms of both signs
+        r = rel_err(g, o, 1e-5)
+        assert np.mean(r < 1e-3) > 0.99 and r.max() < 1e-2
 
 
 def test_multi_frequency_and_camera_batch(tracer, dump128):
@@ -130,7 +132,8 @@ def test_3d_phi_independent_dump_equals_2d(tracer, dump128):
     d3 = Dump(grid=g3, prims=np.repeat(dump128.prims, 6, axis=3))
     tracer.load_dump(d3)
     img3, _ = tracer.render([INCL], [FREQ])
-    assert rel_err(img3[0, :, 0], img2[0, :, 0]).max() < 1e-9
+    # identical up to the FP32 rounding of (1 - d3) w + d3 w in the sampler
+    assert rel_err(img3[0, :, 0], img2[0, :, 0]).max() < 1e-5
 
 
 def test_3d_dump_matches_oracle(tracer):
@@ -219,6 +222,8 @@ def test_probe_camera(tracer, dump128):
 
 
 def test_probe_emission(tracer):
+    """FP32 emissivity and absorptivity of the transfer pass against the FP64
+    oracle functions."""
     rng = np.random.default_rng(4)
     n = 300
     q = np.column_stack([10 ** rng.uniform(-1, 3, n), rng.uniform(-1, 1, n),
@@ -229,9 +234,15 @@ def test_probe_emission(tracer):
     for i in range(n):
         j = lib.or_j_thsynch(q[i, 0], q[i, 1], q[i, 2], q[i, 3], q[i, 4])
         a = lib.or_a_th(j, q[i, 3], q[i, 2])
-        assert out[i, 0] == pytest.approx(j, rel=1e-12, abs=1e-300)
+        # FP32 exp(-X^(1/3)) carries a relative error ~ X^(1/3) * 2^-24
+        nuc = 4.8032068e-10 * q[i, 0] / (2 * np.pi * 9.1093826e-28 * 2.99792458e10)
+        X = q[i, 3] / ((2.0 / 9.0) * nuc * q[i, 2] ** 2 * np.sqrt(1 - q[i, 1] ** 2))
+        tol = 2e-5 + 4e-7 * np.cbrt(X)
+        assert out[i, 0] == pytest.approx(j, rel=tol, abs=1e-300)
         if np.isfinite(a):
-            assert out[i, 1] == pytest.approx(a, rel=1e-10, abs=1e-300)
+            assert out[i, 1] == pytest.approx(a, rel=tol, abs=1e-300)
+        else:
+            assert np.isnan(out[i, 1]) == np.isnan(a)
 
 
 def test_fluid_params_hook(tracer, dump128):
@@ -245,12 +256,15 @@ def test_fluid_params_hook(tracer, dump128):
     for i in range(len(X)):
         ref = O.fluid_at(P, dump128.prims, X[i])
         assert bool(out[i, 11]) == ref["ok"]
-        np.testing.assert_allclose(out[i, 3:7], ref["ucon"], rtol=1e-11, atol=1e-14)
-        np.testing.assert_allclose(out[i, 7:11], ref["bcon"], rtol=1e-9, atol=1e-14)
+        # the sampler runs in FP32 from FP32 primitives
+        np.testing.assert_allclose(out[i, 3:7], ref["ucon"], rtol=2e-6,
+                                   atol=2e-6 * np.abs(ref["ucon"]).max())
+        np.testing.assert_allclose(out[i, 7:11], ref["bcon"], rtol=1e-4,
+                                   atol=1e-4 * np.abs(ref["bcon"]).max() + 1e-30)
         if ref["ok"]:
-            assert out[i, 0] == pytest.approx(ref["ne"], rel=1e-12)
-            assert out[i, 1] == pytest.approx(ref["thetae"], rel=1e-10)
-            assert out[i, 2] == pytest.approx(ref["B"], rel=1e-9)
+            assert out[i, 0] == pytest.approx(ref["ne"], rel=1e-6)
+            assert out[i, 1] == pytest.approx(ref["thetae"], rel=1e-5)
+            assert out[i, 2] == pytest.approx(ref["B"], rel=1e-4)
 
 
 # ---- error paths ----------------------------------------------------------

@@ -61,11 +61,17 @@ def test_host_build_matches_oracle_image_and_counters(dump128):
     P = O.params(dump128.grid, width=W, height=W, inclination=INCL, M_unit=M_UNIT,
                  R_high=R_HIGH)
     oimg, odimg, ocnt = O.render(P, dump128.prims, [FREQ], grad=True, counters=True)
-    assert rel_err(img[:, 0], oimg[:, 0]).max() < 1e-8
+    # FP32 transfer against the FP64 oracle (the geodesics are FP64 in both)
+    assert rel_err(img[:, 0], oimg[:, 0]).max() < 1e-4
+    assert abs(img[:, 0].sum() / oimg[:, 0].sum() - 1) < 1e-6
     assert np.array_equal(cnt[:, 0], ocnt[:, 0]) and np.array_equal(cnt[:, 1], ocnt[:, 2])
     d = dimg.reshape(-1, 2)
     for q in range(2):
-        assert rel_err(d[:, q], odimg[:, 0, q], 1e-5).max() < 1e-7
+        # a pixel's gradient is a sum of terms of both signs; FP32 rounding of
+        # the terms shows up relative to a small net value on optically thick rays
+        r = rel_err(d[:, q], odimg[:, 0, q], 1e-5)
+        assert np.mean(r < 1e-3) > 0.99 and r.max() < 1e-2
+        assert d[:, q].sum() / odimg[:, 0, q].sum() == pytest.approx(1.0, abs=1e-4)
 
 
 def test_host_build_multi_frequency(dump128):
@@ -76,4 +82,4 @@ def test_host_build_multi_frequency(dump128):
                  R_high=R_HIGH)
     oimg = O.render(P, dump128.prims, freqs)
     for f in range(3):
-        assert rel_err(img[:, f], oimg[:, f]).max() < 1e-8
+        assert rel_err(img[:, f], oimg[:, f]).max() < 1e-4
