@@ -68,6 +68,7 @@ struct Params {
     int n1, n2, n3;
     double startx1, startx2, startx3;
     double dx1, dx2, dx3;
+    double idx1, idx2, idx3;     // 1 / dx
     double stopx1, stopx2;
     double a, hslope;
     // camera and integrator
@@ -83,7 +84,7 @@ struct Params {
     double freq[kMaxFreq];
     // FP32 copies for the transfer
     float fnu[kMaxFreq];
-    float fNe_unit, fB_unit, fR_high, fR_low, fa, fpi_h;   // fpi_h = (1 - h) pi
+    float fNe_unit, fB_unit, fiB_unit, fR_high, fR_low, fa, fpi_h;   // fpi_h = (1 - h) pi
     int ncam;
     Camera cam[kMaxCam];
 };
@@ -95,13 +96,29 @@ struct Point {
     double s2, c2; // sin, cos of 2 pi X2
 };
 
+// Single-instruction FP32 reciprocal, square root, exp2 and log2 (about 1
+// ulp) for the transfer pass, whose error budget is 1e-3 per pixel.  The
+// __host__ overloads serve CPU builds of these functions for testing.
+__device__ inline float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ inline float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ inline float flog2(float x) { return __builtin_amdgcn_logf(x); }
+__host__ inline float frcp(float x) { return 1.f / x; }
+__host__ inline float fsqrt(float x) { return sqrtf(x); }
+__host__ inline float fexp2(float x) { return exp2f(x); }
+__host__ inline float flog2(float x) { return log2f(x); }
+__host__ __device__ inline float fexp(float x) { return fexp2(1.4426950408889634f * x); }
+
+// theta = pi X2 + (1-h)/2 sin(2 pi X2) is evaluated as pi * (X2 + (1-h)/(2 pi)
+// sin(2 pi X2)), so both sine/cosine pairs take arguments in units of pi and
+// reduce exactly (sincospi) instead of through a general argument reduction.
 __host__ __device__ inline Point make_point(double X1, double X2, double h)
 {
     Point p;
     p.r = exp(X1);
-    sincos(2. * kPi * X2, &p.s2, &p.c2);
-    double th = kPi * X2 + 0.5 * (1. - h) * p.s2;
-    sincos(th, &p.s, &p.c);
+    sincospi(2. * X2, &p.s2, &p.c2);
+    const double tp = X2 + (1. - h) * (0.5 / kPi) * p.s2;
+    sincospi(tp, &p.s, &p.c);
     return p;
 }
 
@@ -150,17 +167,21 @@ __host__ __device__ inline void geodesic_accel(const Point &p, double a, double 
     const double c = p.c, s = p.s;
     const double cos2 = c * c, sin2 = s * s;
     const double s2th = 2. * s * c;
-    const double cot = c / s;
     const double a2 = a * a;
     const double Sig = r2 + a2 * cos2;
     const double Del = r2 - 2. * r + a2;
     const double Acap = Sig * Del + 2. * r * (r2 + a2);
-    const double iS = 1. / Sig;
+    // the four reciprocals 1/Sigma, 1/sin, 1/r, 1/theta' from one division
+    const double sS = s * Sig, rt = r * thp;
+    const double inv = 1. / (sS * rt);
+    const double iS = inv * (s * rt);
+    const double is = inv * (Sig * rt);
+    const double irp = inv * (sS * thp);
+    const double ithp = inv * (sS * r);
+    const double cot = c * is;
     const double iS2 = iS * iS, iS3 = iS2 * iS;
     const double B = (2. * r2 - Sig) * iS3;
     const double C = r * iS - a2 * B * sin2;
-    const double irp = 1. / rp;
-    const double ithp = 1. / thp;
 
     // products k^a k^b, off-diagonal ones doubled
     const double k00 = k[0] * k[0], k11 = k[1] * k[1], k22 = k[2] * k[2], k33 = k[3] * k[3];
@@ -192,7 +213,7 @@ __host__ __device__ inline void geodesic_accel(const Point &p, double a, double 
     A[1] = -(G100 * k00 + G101 * k01 + G103 * k03 + G111 * k11 + G112 * k12 + G113 * k13 +
              G122 * k22 + G133 * k33);
     // Gamma^theta (Gamma^2_02 = Gamma^2_23 = 0)
-    const double G200 = iS * G002 / thp2;
+    const double G200 = iS * G002 * (ithp * ithp);
     const double G201 = G200 * rp;
     const double G203 = a * r * (r2 + a2) * s2th * iS3 * ithp;
     const double G211 = G200 * rp2;
@@ -217,17 +238,17 @@ __host__ __device__ inline void geodesic_accel(const Point &p, double a, double 
              G313 * k13 + G322 * k22 + G323 * k23 + G333 * k33);
 }
 
-// integrator.c:109-122 (kept literally, including the 1e-40 guards)
+// integrator.c:109-122: dl = -max(1 / sum_i 1/dl_i, 1e-12) with
+// dl_1 = eps/|k^1|, dl_2 = eps min(X2, 1-X2)/|k^2|, dl_3 = eps/|k^3|, written
+// as eps / (|k^1| + |k^2|/min(X2, 1-X2) + |k^3|).  The reference's 1e-40
+// guards change the result only when a component or min(X2, 1-X2) is
+// exactly 0; both forms then give the 1e-12 floor (a NaN quotient from 0/0
+// also falls to the floor through fmax).
 __host__ __device__ inline double stepsize(const double y[8], double eps)
 {
-    const double S2 = kSmall * kSmall;
-    double dlx1 = eps / (fabs(y[5]) + S2);
-    double dlx2 = eps * fmin(y[2], 1. - y[2]) / (fabs(y[6]) + S2);
-    double dlx3 = eps / (fabs(y[7]) + S2);
-    double i1 = 1. / (fabs(dlx1) + S2);
-    double i2 = 1. / (fabs(dlx2) + S2);
-    double i3 = 1. / (fabs(dlx3) + S2);
-    return -fmax(1. / (i1 + i2 + i3), 1e-12);
+    const double m = fabs(fmin(y[2], 1. - y[2]));
+    const double sum = fabs(y[5]) + fabs(y[6]) / m + fabs(y[7]);
+    return -fmax(eps / sum, 1e-12);
 }
 
 // One RK2 midpoint step (integrator.c:44-70).  `pt` holds the trigonometry of
@@ -280,8 +301,12 @@ template <bool DIM3>
 __host__ __device__ inline void sample_prims(const Params &P, const float4 *__restrict__ prims,
                                              const double y[8], float v[8])
 {
-    int i = (int)((y[1] - P.startx1) / P.dx1 - 0.5);
-    int j = (int)((y[2] - P.startx2) / P.dx2 - 0.5);
+    // grmhd.c:46-71: i = (int)(t - 1/2) truncates toward zero, the clamps set
+    // del to 0 / 1, and otherwise del = t - (i + 1/2) with t = (X - x0)/dx
+    const double t1 = (y[1] - P.startx1) * P.idx1;
+    const double t2 = (y[2] - P.startx2) * P.idx2;
+    int i = (int)(t1 - 0.5);
+    int j = (int)(t2 - 0.5);
     double d1, d2;
     if (i < 0) {
         i = 0;
@@ -290,7 +315,7 @@ __host__ __device__ inline void sample_prims(const Params &P, const float4 *__re
         i = P.n1 - 2;
         d1 = 1.;
     } else {
-        d1 = (y[1] - ((i + 0.5) * P.dx1 + P.startx1)) / P.dx1;
+        d1 = t1 - (i + 0.5);
     }
     if (j < 0) {
         j = 0;
@@ -299,10 +324,11 @@ __host__ __device__ inline void sample_prims(const Params &P, const float4 *__re
         j = P.n2 - 2;
         d2 = 1.;
     } else {
-        d2 = (y[2] - ((j + 0.5) * P.dx2 + P.startx2)) / P.dx2;
+        d2 = t2 - (j + 0.5);
     }
-    const float c0 = (float)((1. - d1) * (1. - d2)), c1 = (float)((1. - d1) * d2);
-    const float c2 = (float)(d1 * (1. - d2)), c3 = (float)(d1 * d2);
+    const float f1 = (float)d1, f2 = (float)d2;
+    const float c0 = (1.f - f1) * (1.f - f2), c1 = (1.f - f1) * f2;
+    const float c2 = f1 * (1.f - f2), c3 = f1 * f2;
 #define RT_BI(f00, f01, f10, f11) ((f00) * c0 + (f01) * c1 + (f10) * c2 + (f11) * c3)
     if (!DIM3) {
         const size_t b00 = (size_t)(i * P.n2 + j) * 2;
@@ -325,7 +351,7 @@ __host__ __device__ inline void sample_prims(const Params &P, const float4 *__re
         double ph = fmod(y[3] - P.startx3, period);
         if (ph < 0.)
             ph += period;
-        double t = ph / P.dx3 - 0.5;
+        double t = ph * P.idx3 - 0.5;
         double fk = floor(t);
         const float d3 = (float)(t - fk);
         int k = (int)fk;
@@ -379,7 +405,7 @@ __host__ __device__ inline MetricF metric_f32(const Point &pt, const Params &P)
     const float sin2 = s * s;
     const float rho2 = r * r + a * a * c * c;
     const float hfac = P.fpi_h * (float)pt.c2 + 3.14159265f;
-    const float irho2 = 1.f / rho2;
+    const float irho2 = frcp(rho2);
     const float z = 2.f * r * irho2;
     g.g00 = -1.f + z;
     g.g01 = z * r;
@@ -421,7 +447,7 @@ __host__ __device__ inline bool fluid_sample(const Params &P, const float4 *__re
     const float V1 = v[2], V2 = v[3], V3 = v[4];
     float VdotV = g.g11 * V1 * V1 + g.g13 * V1 * V3 + g.g22 * V2 * V2 + g.g13 * V3 * V1 +
                   g.g33 * V3 * V3;
-    float Vfac = sqrtf(-1.f / g.gcon00 * (1.f + fabsf(VdotV)));
+    float Vfac = fsqrt(-frcp(g.gcon00) * (1.f + fabsf(VdotV)));
     float u[4], ud[4], b[4], bd[4];
     u[0] = -Vfac * g.gcon00;
     u[1] = V1 - Vfac * g.gcon01;
@@ -429,7 +455,7 @@ __host__ __device__ inline bool fluid_sample(const Params &P, const float4 *__re
     u[3] = V3;
     lower_f(g, u, ud);
     float UdotBp = ud[1] * v[5] + ud[2] * v[6] + ud[3] * v[7];
-    const float iu0 = 1.f / u[0];
+    const float iu0 = frcp(u[0]);
     b[0] = UdotBp;
     b[1] = (v[5] + u[1] * UdotBp) * iu0;
     b[2] = (v[6] + u[2] * UdotBp) * iu0;
@@ -437,17 +463,18 @@ __host__ __device__ inline bool fluid_sample(const Params &P, const float4 *__re
     lower_f(g, b, bd);
     float Bsq = b[0] * bd[0] + b[1] * bd[1] + b[2] * bd[2] + b[3] * bd[3];
     F.ne = rho * P.fNe_unit;
-    const float bnorm = sqrtf(Bsq);
+    const float bnorm = fsqrt(Bsq);
     F.B = bnorm * P.fB_unit + 1e-20f;
-    float beta = uu * (1.f / 3.f) / (0.5f * (Bsq + 1e-20f));
+    float beta = uu * (2.f / 3.f) * frcp(Bsq + 1e-20f);
     float b2 = beta * beta;
-    float w = 1.f / (1.f + b2);
+    float w = frcp(1.f + b2);
     float trat = P.fR_high + (P.fR_low - P.fR_high) * w;
-    float itr2 = 1.f / (trat + 2.f);
+    float itr2 = frcp(trat + 2.f);
     float gam2 = 0.5f * ((1.f + (2.f / 3.f) * (trat + 1.f) * itr2) + 4.f / 3.f);
     const float mr = (float)(kProtonMass / kElectronMass);
-    const float itr1 = 1.f / (1.f + trat);
-    const float uor = uu / rho;
+    const float itr1 = frcp(1.f + trat);
+    const float irho = frcp(rho);
+    const float uor = uu * irho;
     F.thetae = uor * ((gam2 - 1.f) * mr * itr1);
     float dgam2 = (1.f / 3.f) * itr2 * itr2;
     const float dtrat = (b2 <= 3e38f) ? b2 * w : 1.f;   // d trat / d R_high
@@ -467,15 +494,15 @@ __host__ __device__ inline bool fluid_sample(const Params &P, const float4 *__re
         beta_bern_out[0] = beta;
         beta_bern_out[1] = -(1.f + uor * (4.f / 3.f)) * ud[0];
     }
-    if (Bsq / rho > 1.f)
+    if (Bsq * irho > 1.f)
         return false;
     // pitch angle (cos) and Doppler factor with k lowered by the same metric
     float k[4] = {(float)y[4], (float)y[5], (float)y[6], (float)y[7]};
     float kd[4];
     lower_f(g, k, kd);
     float ku = kd[0] * u[0] + kd[1] * u[1] + kd[2] * u[2] + kd[3] * u[3];
-    float mu = (kd[0] * b[0] + kd[1] * b[1] + kd[2] * b[2] + kd[3] * b[3]) /
-               (fabsf(ku) * (F.B / P.fB_unit));
+    float mu = (kd[0] * b[0] + kd[1] * b[1] + kd[2] * b[2] + kd[3] * b[3]) *
+               frcp(fabsf(ku) * (F.B * P.fiB_unit));
     if (fabsf(mu) > 1.f)
         mu /= fabsf(mu);
     F.mu = mu;
@@ -502,10 +529,10 @@ struct EmisF {
 __host__ __device__ inline EmisF emission_f32(float B, float mu, float Te, float nu_p, float ne)
 {
     EmisF e;
-    const float sth = sqrtf(1.f - mu * mu);
+    const float sth = fsqrt(1.f - mu * mu);
     const float nu_c = (float)(kElectronCharge / (2. * kPi * kElectronMass * kC)) * B;
     const float nu_s = (2.f / 9.f) * nu_c * Te * Te * sth;
-    const float X = nu_p / nu_s;
+    const float X = nu_p * frcp(nu_s);
     e.X = X;
     e.poison = false;
     if (nu_p == 1.f || fabsf(sth) < 1.e-10f || !(Te > 1e-4f)) {
@@ -518,13 +545,15 @@ __host__ __device__ inline EmisF emission_f32(float B, float mu, float Te, float
         }
         return e;
     }
-    const float cb = cbrtf(X);
-    const float ff = sqrtf(X) + 1.8877486253633868f * sqrtf(cb);   // 2^(11/12) X^(1/6)
+    const float lx = flog2(X);
+    const float cb = fexp2(lx * (1.f / 3.f));                        // X^(1/3)
+    const float ff = fexp2(0.5f * lx) + 1.8877486253633868f * fexp2(lx * (1.f / 6.f));
     e.cb = cb;
     e.jh = ne * (float)(1.4142135623730951 * kPi / 3. / 9.) * (nu_c * sth) * (ff * ff);
-    const float x = (float)(kPlanck / (kElectronMass * kC * kC)) * nu_p / Te;
-    const float cn = (float)kC / nu_p;
-    e.kap = (float)(0.5 * kE2CH) * expm1f(x) * (cn * cn) / nu_p;
+    const float inu = frcp(nu_p);
+    const float x = (float)(kPlanck / (kElectronMass * kC * kC)) * nu_p * frcp(Te);
+    const float cn = (float)kC * inu;
+    e.kap = (float)(0.5 * kE2CH) * expm1f(x) * (cn * cn) * inu;
     return e;
 }
 
@@ -532,8 +561,8 @@ __host__ __device__ inline EmisF emission_f32(float B, float mu, float Te, float
 __host__ __device__ inline float dlnj_dlnX(float X, float cb)
 {
     const float c1112 = 1.8877486253633868f;
-    float sx = sqrtf(X), t = sqrtf(cb);
-    return 2.f * (0.5f * sx + c1112 / 6.f * t) / (sx + c1112 * t) - cb * (1.f / 3.f);
+    float sx = fsqrt(X), t = fsqrt(cb);
+    return 2.f * (0.5f * sx + c1112 / 6.f * t) * frcp(sx + c1112 * t) - cb * (1.f / 3.f);
 }
 
 // Per-ray accumulators for NF frequencies: optical depth and I_nu nu^3
@@ -583,21 +612,21 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
         float nu_p = g * P.fnu[f];
         if (nu_p <= 0.f) {
             nu_p = 1.f;
-            g = 1.f / P.fnu[f];
+            g = frcp(P.fnu[f]);
         }
         const EmisF e = emission_f32(F.B, F.mu, F.thetae, nu_p, F.ne);
-        const float jx = e.jh * expf(-e.cb);
+        const float jx = e.jh * fexp(-e.cb);
         float dtau = e.jh != 0.f ? g * dl * jx * e.kap : (e.poison ? NAN : 0.f);
         kahan_add(acc.tau[f], acc.ctau[f], dtau);
         const float tau = acc.tau[f];
-        const float dI = (float)kE2C * jx * expf(-tau) * (dl / (g * g));
+        const float dI = (float)kE2C * jx * fexp(-tau) * (dl * frcp(g * g));
         float dlnj0 = 0.f, dlnj1 = 0.f, dlna1 = 0.f;
         if (GRAD && e.jh != 0.f) {
             float Dx = dlnj_dlnX(e.X, e.cb);
-            float dlnB = 0.5f * (F.B - 1e-20f) / F.B;
-            float dlnT = F.dthetae / F.thetae;
-            float x = (float)(kPlanck / (kElectronMass * kC * kC)) * nu_p / F.thetae;
-            float dlnBnu = x / (-expm1f(-x));
+            float dlnB = 0.5f * (F.B - 1e-20f) * frcp(F.B);
+            float dlnT = F.dthetae * frcp(F.thetae);
+            float x = (float)(kPlanck / (kElectronMass * kC * kC)) * nu_p * frcp(F.thetae);
+            float dlnBnu = x * frcp(-expm1f(-x));
             dlnj0 = 1.f + dlnB * (1.f - Dx);
             dlnj1 = -2.f * Dx * dlnT;
             dlna1 = dlnj1 - dlnBnu * dlnT;
@@ -758,6 +787,9 @@ inline const char *fill_params(const RtParams &p, const RtGrid &g, const double 
     P.dx1 = g.dx1;
     P.dx2 = g.dx2;
     P.dx3 = g.dx3;
+    P.idx1 = 1. / g.dx1;
+    P.idx2 = 1. / g.dx2;
+    P.idx3 = g.n3 > 1 ? 1. / g.dx3 : 0.;
     P.stopx1 = g.startx1 + g.n1 * g.dx1;
     P.stopx2 = g.startx2 + g.n2 * g.dx2;
     P.a = g.a;
@@ -780,6 +812,7 @@ inline const char *fill_params(const RtParams &p, const RtGrid &g, const double 
     P.R_low = p.R_low;
     P.fNe_unit = (float)P.Ne_unit;
     P.fB_unit = (float)P.B_unit;
+    P.fiB_unit = (float)(1. / P.B_unit);
     P.fR_high = (float)p.R_high;
     P.fR_low = (float)p.R_low;
     P.fa = (float)g.a;

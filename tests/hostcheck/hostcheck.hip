@@ -5,6 +5,19 @@
 // not-gpu test suite can compare the kernel's arithmetic with the oracle
 // before any GPU run.  This library is test infrastructure: the product
 // (libraptor_rt.so) has no CPU path.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+// Host overload of the device library's sincospi for this CPU build (the
+// kernel uses the device version): sin(pi x), cos(pi x) after reducing x to
+// [-1, 1], accurate to a few ulp, which is all the comparison needs.
+static inline __host__ void sincospi(double x, double *s, double *c)
+{
+    double r = x - 2. * nearbyint(0.5 * x);
+    *s = sin(M_PI * r);
+    *c = cos(M_PI * r);
+}
+
 #include "../../include/raptor_rt.h"
 #include "../../raptor_amd/csrc/rt_physics.hpp"
 
