@@ -2,12 +2,13 @@
 """bench.py -- pixel-geodesics/s of the per-pixel ray tracer on one MI355X.
 
 One step = one render of the 1024x1024 image of a 256x256 HARM torus dump at
-230 GHz (inclination 60 deg, 40 M field of view, FP64 geodesics and
+230 GHz (inclination 60 deg, 40 M field of view; FP64 geodesics, FP32
 transfer): every pixel's null geodesic plus its radiative transfer.  Inputs
 are resident in HBM before the timed region; outputs stay in HBM.
 
 Prints one JSON line (rank 0).  Extra fields:
-  roofline      algorithmic FLOP rate of the trace kernel vs the FP64 vector peak
+  roofline      algorithmic FLOP rate of the trace kernel vs the FP64/FP32
+                vector peak for its mix of work
   grad          image + d/d(ln M_unit, R_high) throughput beside plain images
   cpu_baseline  the FP64 CPU oracle (OpenMP, all host cores) on a pixel sample
   parity        GPU vs oracle on that same sample
@@ -30,6 +31,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "pixel-geodesics/s at 1024^2, 230 GHz; image+grad/s vs plain, <=1e-4 flux err"
 FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector (dense) peak
+FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 vector (dense) peak
 
 WORKLOAD = dict(workload="harm2d-torus-256x256_img1024x1024_230GHz_i60", dump="256x256x1",
                 image="1024x1024", freq_hz=2.3e11, inclination_deg=60.0, M_unit=1e23,
@@ -85,8 +87,10 @@ def main():
     tr.render_device(incl, freq, d_img, None, d_cnt)
     tr.synchronize()
     cnt = tr.d2h(d_cnt, (npix, 2), np.int32)
-    alg_flops = flops.image_flops(cnt[:, 0], cnt[:, 1], 1)
+    f64, f32 = flops.image_flops_split(cnt[:, 0], cnt[:, 1], 1)
+    alg_flops = f64 + f32
     alg_flops_grad = flops.image_flops(cnt[:, 0], cnt[:, 1], 1, grad=True)
+    peak = flops.mix_peak(f64, f32, FP64_PEAK_TFLOPS, FP32_PEAK_TFLOPS)
 
     for _ in range(args.warmup):
         tr.render_device(incl, freq, d_img)
@@ -118,6 +122,12 @@ def main():
     img = tr.d2h(d_img, (npix,), np.float64)
     jy = tr.jansky_factor()
 
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "trace_kernel_traffic.json")
+    if W == 1024 and os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+
     result = {
         "metric": METRIC,
         "value": value,
@@ -129,7 +139,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64+f32",
         "data": "synthetic (seeded Fishbone-Moncrief torus, HARM MKS, a=0.9375)",
         "config": dict(WORKLOAD, image="%dx%d" % (W, W)),
         "images_per_s": 1e3 / ms_per_step,
@@ -137,13 +147,18 @@ def main():
         "roofline": {
             "bound": "valu",
             "achieved": achieved,
-            "peak": FP64_PEAK_TFLOPS,
+            "peak": peak,
             "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "frac": achieved / peak,
+            "traffic": traffic,
+            "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE, "
+                            "copy-calibrated (profiles/trace_kernel_traffic.json)",
             "kernel": "trace_kernel<1,false,false>",
             "kernel_ms": kern_ms,
             "alg_flops_per_launch": alg_flops,
+            "alg_flops_fp64": f64,
+            "alg_flops_fp32": f32,
+            "peak_note": "FP64/FP32 mix-weighted vector peak (78.6 / 157.3 TFLOP/s)",
             "rk2_steps_per_launch": int(cnt[:, 0].sum()),
             "samples_per_launch": int(cnt[:, 1].sum()),
         },

@@ -12,26 +12,46 @@ evaluations).  Each add, subtract or multiply counts 1 (an FMA 2), and each
 division, square/cube root, exp, sin or cos counts 1; products of literal
 constants are folded.  Per-function breakdown:
 
-  make_point            6 arith + 5 transcendental            = 11
-  geodesic_accel      235 arith + 5 division                  = 240
-  stepsize             10 arith + 7 division                  = 17
-  rk2_step            2*accel + 2*make_point + 40 updates     = 542
-  loop bookkeeping      1                                      = 1
-  F_STEP = 17 + 542 + 1                                        = 560
-    of which 10 transcendental, 17 division
+  make_point            5 arith + 5 transcendental (exp, 2 sincospi) = 10
+  geodesic_accel      249 arith + 1 division                         = 250
+  stepsize              3 arith + 2 division                         = 5
+  rk2_step            2*accel + 2*make_point + 40 updates            = 560
+  loop bookkeeping      1                                             = 1
+  F_STEP = 5 + 560 + 1                                                = 566
+    of which 10 transcendental, 4 division
 
-  fluid_sample        226 arith + 20 division + 2 sqrt         = 248 (F_FLUID)
-  transfer per freq    40 arith + 10 division + 7 transc.      = 57  (F_NU)
-  tangent pass: +15 per sample (dTheta_e/dR_high), +41 per sample-frequency
+  fluid_sample (FP32) 231 arith + 9 reciprocal + 2 sqrt              = 242 (F_FLUID)
+  transfer per freq    46 arith + 4 reciprocal + 1 sqrt + 7 transc.  = 58  (F_NU)
+  tangent pass: +11 per sample (dTheta_e/dR_high), +41 per sample-frequency
 """
 
-F_STEP = 560
+F_STEP = 566
 F_STEP_TRANS = 10
-F_STEP_DIV = 17
-F_FLUID = 248
-F_NU = 57
-F_FLUID_GRAD = 15
+F_STEP_DIV = 4
+F_FLUID = 242
+F_NU = 58
+F_FLUID_GRAD = 11
 F_NU_GRAD = 41
+
+
+def image_flops_split(steps, samples, n_freq, grad=False):
+    """(FP64 FLOPs, FP32 FLOPs): the geodesic steps run in FP64, the fluid
+    sample and transfer in FP32 (their FP64 tangent accumulators are small
+    and counted with the FP32 part)."""
+    import numpy as np
+    ns = float(np.sum(steps))
+    nr = float(np.sum(samples))
+    f64 = ns * F_STEP
+    f32 = nr * (F_FLUID + n_freq * F_NU)
+    if grad:
+        f32 += nr * (F_FLUID_GRAD + n_freq * F_NU_GRAD)
+    return f64, f32
+
+
+def mix_peak(f64, f32, peak64, peak32):
+    """Peak rate for a mix of FP64 and FP32 work: total work over the time it
+    takes when each part runs at its own peak."""
+    return (f64 + f32) / (f64 / peak64 + f32 / peak32)
 
 
 def image_flops(steps, samples, n_freq, grad=False):
