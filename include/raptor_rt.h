@@ -132,12 +132,15 @@ double rt_jansky_factor(const rt_ctx *ctx);
 
 /* ---- launch configuration ------------------------------------------------ */
 
-/* Threads per block and resident blocks per CU of the persistent trace
- * kernel (0 = library default), and rays fetched per queue claim. */
+/* Launch shape of the persistent trace kernel.  block: threads per
+ * workgroup (64; other values are rejected).  blocks_per_cu: resident
+ * workgroups per CU (0 = the occupancy limit).  refill: a lane whose ray has
+ * ended claims a new one only when at least `refill` lanes of its wave are
+ * idle (1..64; 0 = library default). */
 typedef struct {
     int block;
     int blocks_per_cu;
-    int chunk;
+    int refill;
 } rt_launch_config;
 
 int rt_set_launch_config(rt_ctx *ctx, const rt_launch_config *cfg);

@@ -24,56 +24,85 @@
 
 using namespace rt;
 
+static constexpr int kDefaultRefill = 1;
+
 namespace {
 
 struct TraceArgs {
     const float4 *prims;
     unsigned int *queue;
     int total;
-    int chunk;
+    int refill;
     double *image;
     double *dimage;
     rt_ray_counters *counters;
 };
 
 template <int NF, bool GRAD, bool DIM3>
+__device__ inline void write_ray(const Params &P, const TraceArgs &A, int ray,
+                                 const Ray<NF, GRAD> &R)
+{
+    const size_t o = (size_t)ray * P.nf;
+    for (int f = 0; f < NF; f++) {
+        if (f < P.nf) {
+            A.image[o + f] = (double)R.acc.I[f];
+            if (GRAD) {
+                A.dimage[(o + f) * 2 + 0] = R.acc.dI[f][0];
+                A.dimage[(o + f) * 2 + 1] = R.acc.dI[f][1];
+            }
+        }
+    }
+    if (A.counters) {
+        rt_ray_counters c;
+        c.steps = R.steps;
+        c.samples = R.samples;
+        A.counters[ray] = c;
+    }
+}
+
+// Each lane traces one ray at a time.  A lane whose ray has ended waits until
+// at least `refill` lanes of its wave are idle; then all idle lanes claim new
+// ray ids together.  Lanes claiming together get consecutive ids (the atomic
+// is combined across the wave), i.e. neighbouring pixels, whose rays stay in
+// step for most of their length; refill = 1 refills each lane at once (best
+// balance), refill = 64 refills whole waves (most coherence).  The loop ends
+// when the queue is exhausted and every lane is idle, so every wave drains.
+template <int NF, bool GRAD, bool DIM3>
 __global__ __launch_bounds__(64) void trace_kernel(const Params P, const TraceArgs A)
 {
     Ray<NF, GRAD> R;
     const int total = A.total;
-    int ray = (int)atomicAdd(A.queue, (unsigned)A.chunk);
-    int left = ray < total ? min(A.chunk, total - ray) : 0;
-    if (left > 0)
+    int ray = (int)atomicAdd(A.queue, 1u);
+    bool active = ray < total;
+    bool exhausted = !active;
+    if (active)
         ray_init<NF, GRAD>(P, ray / P.npix, ray % P.npix, R);
-    while (left > 0) {
-        bool cont = ray_alive<NF, GRAD>(P, R);
-        if (cont)
-            cont = ray_step<NF, GRAD, DIM3>(P, A.prims, R);
-        if (!cont) {
-            const size_t o = (size_t)ray * P.nf;
-            for (int f = 0; f < NF; f++) {
-                if (f < P.nf) {
-                    A.image[o + f] = (double)R.acc.I[f];
-                    if (GRAD) {
-                        A.dimage[(o + f) * 2 + 0] = R.acc.dI[f][0];
-                        A.dimage[(o + f) * 2 + 1] = R.acc.dI[f][1];
-                    }
+    while (true) {
+        if (active) {
+            bool cont = ray_alive<NF, GRAD>(P, R);
+            if (cont)
+                cont = ray_step<NF, GRAD, DIM3>(P, A.prims, R);
+            if (!cont) {
+                write_ray<NF, GRAD, DIM3>(P, A, ray, R);
+                active = false;
+            }
+        }
+        const unsigned long long act = __ballot(active);
+        exhausted = __ballot(exhausted) != 0ull;
+        const int idle = 64 - __popcll(act);
+        if (exhausted) {
+            if (act == 0ull)
+                break;
+        } else if (idle >= A.refill || (act == 0ull)) {
+            if (!active) {
+                ray = (int)atomicAdd(A.queue, 1u);
+                if (ray < total) {
+                    ray_init<NF, GRAD>(P, ray / P.npix, ray % P.npix, R);
+                    active = true;
+                } else {
+                    exhausted = true;
                 }
             }
-            if (A.counters) {
-                rt_ray_counters c;
-                c.steps = R.steps;
-                c.samples = R.samples;
-                A.counters[ray] = c;
-            }
-            ray++;
-            left--;
-            if (left == 0) {
-                ray = (int)atomicAdd(A.queue, (unsigned)A.chunk);
-                left = ray < total ? min(A.chunk, total - ray) : 0;
-            }
-            if (left > 0)
-                ray_init<NF, GRAD>(P, ray / P.npix, ray % P.npix, R);
         }
     }
 }
@@ -214,7 +243,7 @@ struct rt_ctx {
     double *d_image = nullptr, *d_dimage = nullptr;
     size_t image_cap = 0, dimage_cap = 0;
     double last_ms = 0.;
-    rt_launch_config cfg{64, 0, 1};
+    rt_launch_config cfg{64, 0, kDefaultRefill};
     char err[512] = {0};
 };
 
@@ -283,7 +312,7 @@ static hipError_t launch_trace(rt_ctx *c, const Params &P, const TraceArgs &A)
         per_cu = std::max(occ, 1);
     }
     long long grid = (long long)per_cu * c->num_cu;
-    long long need = ((long long)A.total + A.chunk - 1) / A.chunk;
+    long long need = ((long long)A.total + 63) / 64;
     grid = std::max(1LL, std::min(grid, need));
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), 0, c->stream, P, A);
     return hipGetLastError();
@@ -331,7 +360,7 @@ static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *fr
     A.prims = c->d_prims;
     A.queue = c->d_queue;
     A.total = ncam * P.npix;
-    A.chunk = std::max(1, c->cfg.chunk);
+    A.refill = std::min(64, std::max(1, c->cfg.refill));
     A.image = d_image;
     A.dimage = grad ? d_dimage : nullptr;
     A.counters = d_counters;
@@ -566,12 +595,13 @@ int rt_set_launch_config(rt_ctx *c, const rt_launch_config *cfg)
 {
     if (!c)
         return RT_E_INVALID;
-    if (!cfg || cfg->blocks_per_cu < 0 || cfg->chunk < 0 || cfg->blocks_per_cu > 64 ||
-        cfg->chunk > 4096)
-        return fail(c, RT_E_INVALID, "bad launch config");
+    if (!cfg || (cfg->block != 0 && cfg->block != 64) || cfg->blocks_per_cu < 0 ||
+        cfg->blocks_per_cu > 64 || cfg->refill < 0 || cfg->refill > 64)
+        return fail(c, RT_E_INVALID, "bad launch config (block 64, blocks_per_cu 0..64, "
+                                     "refill 0..64)");
     c->cfg.block = 64;
     c->cfg.blocks_per_cu = cfg->blocks_per_cu;
-    c->cfg.chunk = cfg->chunk == 0 ? 1 : cfg->chunk;
+    c->cfg.refill = cfg->refill == 0 ? kDefaultRefill : cfg->refill;
     return RT_OK;
 }
 

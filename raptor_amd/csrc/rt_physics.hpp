@@ -252,21 +252,24 @@ __host__ __device__ inline double stepsize(const double y[8], double eps)
 }
 
 // One RK2 midpoint step (integrator.c:44-70).  `pt` holds the trigonometry of
-// y's point on entry and of the new point on exit.
+// y's point on entry and of the new point on exit.  The time coordinate
+// X^0 is not advanced: nothing on the path reads it (the metric is
+// stationary; the reference only prints it in its debug geodesic dump).
 __host__ __device__ inline void rk2_step(double y[8], Point &pt, double dl, double a, double h)
 {
     double A[4], ym[8];
     geodesic_accel(pt, a, h, y + 4, A);
-    for (int i = 0; i < 4; i++) {
+    ym[0] = y[0];
+    for (int i = 1; i < 4; i++)
         ym[i] = y[i] + dl * y[4 + i] * 0.5;
+    for (int i = 0; i < 4; i++)
         ym[4 + i] = y[4 + i] + dl * A[i] * 0.5;
-    }
     Point pm = make_point(ym[1], ym[2], h);
     geodesic_accel(pm, a, h, ym + 4, A);
-    for (int i = 0; i < 4; i++) {
+    for (int i = 1; i < 4; i++)
         y[i] = y[i] + dl * ym[4 + i];
+    for (int i = 0; i < 4; i++)
         y[4 + i] = y[4 + i] + dl * A[i];
-    }
     pt = make_point(y[1], y[2], h);
 }
 
@@ -650,8 +653,8 @@ template <int NF, bool GRAD>
 struct Ray {
     double y[8];
     Point pt;
-    double thetadot_prev;
-    double beta;
+    int thetadot_sign;   // sign of k^theta at the previous step
+    int beta_sign;       // sign of the impact parameter beta
     int steps, turns, samples;
     bool terminate;
     Accum<NF, GRAD> acc;
@@ -674,8 +677,8 @@ __host__ __device__ inline void ray_init(const Params &P, int cam, int pix, Ray<
     pixel_impact(P, pix, alpha, beta);
     init_photon(P.cam[cam], alpha, beta, R.y);
     R.pt = make_point(R.y[1], R.y[2], P.hslope);
-    R.thetadot_prev = 0.;
-    R.beta = beta;
+    R.thetadot_sign = 0;
+    R.beta_sign = (beta > 0.) - (beta < 0.);
     R.steps = 0;
     R.turns = 0;
     R.samples = 0;
@@ -698,10 +701,12 @@ template <int NF, bool GRAD, bool DIM3>
 __host__ __device__ inline bool ray_step(const Params &P, const float4 *__restrict__ prims,
                                          Ray<NF, GRAD> &R)
 {
-    if (R.thetadot_prev * R.y[6] < 0. && R.steps > 2)
+    // integrator.c:243-247: a sign change of k^theta is a turning point
+    const int sgn = (R.y[6] > 0.) - (R.y[6] < 0.);
+    if (R.thetadot_sign * sgn < 0 && R.steps > 2)
         R.turns += 1;
-    R.thetadot_prev = R.y[6];
-    if ((R.beta < 0. && R.turns > kMaxOrder) || (R.beta > 0. && R.turns > kMaxOrder + 1))
+    R.thetadot_sign = sgn;
+    if ((R.beta_sign < 0 && R.turns > kMaxOrder) || (R.beta_sign > 0 && R.turns > kMaxOrder + 1))
         R.terminate = true;
     double dl = stepsize(R.y, P.stepsize);
     rk2_step(R.y, R.pt, dl, P.a, P.hslope);

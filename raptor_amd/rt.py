@@ -44,7 +44,7 @@ class RtParams(ctypes.Structure):
 
 class RtLaunchConfig(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
-                ("chunk", ctypes.c_int)]
+                ("refill", ctypes.c_int)]
 
 
 class RtError(RuntimeError):
@@ -155,9 +155,14 @@ class Tracer:
         self._check(self.lib.rt_load_prims(self.ctx, ctypes.byref(grid),
                                            prims.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
 
-    def set_launch_config(self, blocks_per_cu=0, chunk=1):
-        cfg = RtLaunchConfig(block=64, blocks_per_cu=blocks_per_cu, chunk=chunk)
+    def set_launch_config(self, blocks_per_cu=0, refill=0):
+        cfg = RtLaunchConfig(block=64, blocks_per_cu=blocks_per_cu, refill=refill)
         self._check(self.lib.rt_set_launch_config(self.ctx, ctypes.byref(cfg)))
+
+    def get_launch_config(self):
+        cfg = RtLaunchConfig()
+        self._check(self.lib.rt_get_launch_config(self.ctx, ctypes.byref(cfg)))
+        return dict(block=cfg.block, blocks_per_cu=cfg.blocks_per_cu, refill=cfg.refill)
 
     # -- the pixel loop -----------------------------------------------------------
     def render(self, inclinations, freqs, grad=False):

@@ -204,7 +204,8 @@ def test_probe_accel_and_rk2(tracer, dump128):
         np.testing.assert_allclose(acc[i], A, rtol=1e-11, atol=1e-11 * np.abs(A).max())
         dl = O.stepsize(P, X[i], K[i])
         assert step[i, 0] == pytest.approx(dl, rel=1e-14)
-        np.testing.assert_allclose(step[i, 1:], O.rk2_step(P, y[i], dl), rtol=1e-11,
+        # X^0 (coordinate time) is not advanced by the tracer: nothing reads it
+        np.testing.assert_allclose(step[i, 2:], O.rk2_step(P, y[i], dl)[1:], rtol=1e-11,
                                    atol=1e-12)
 
 
@@ -239,8 +240,11 @@ def test_probe_emission(tracer):
         X = q[i, 3] / ((2.0 / 9.0) * nuc * q[i, 2] ** 2 * np.sqrt(1 - q[i, 1] ** 2))
         tol = 2e-5 + 1e-6 * np.cbrt(X)
         assert out[i, 0] == pytest.approx(j, rel=tol, abs=1e-300)
+        # the oracle's Planck factor exp(x) - 1 (j_nu.c:190) loses ~1e-16/x;
+        # the kernel uses expm1
+        x = 6.6260693e-27 * q[i, 3] / (q[i, 2] * 9.1093826e-28 * 2.99792458e10 ** 2)
         if np.isfinite(a):
-            assert out[i, 1] == pytest.approx(a, rel=tol, abs=1e-300)
+            assert out[i, 1] == pytest.approx(a, rel=tol + 4e-16 / x, abs=1e-300)
         else:
             assert np.isnan(out[i, 1]) == np.isnan(a)
 
