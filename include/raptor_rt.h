@@ -136,15 +136,31 @@ double rt_jansky_factor(const rt_ctx *ctx);
  * workgroup (64; other values are rejected).  blocks_per_cu: resident
  * workgroups per CU (0 = the occupancy limit).  refill: a lane whose ray has
  * ended claims a new one only when at least `refill` lanes of its wave are
- * idle (1..64; 0 = library default). */
+ * idle (1..64; 0 = library default).  tile: rays are handed out in T x T
+ * pixel tiles (1 or 0 = row-major pixel order). */
 typedef struct {
     int block;
     int blocks_per_cu;
     int refill;
+    int tile;
 } rt_launch_config;
 
 int rt_set_launch_config(rt_ctx *ctx, const rt_launch_config *cfg);
 int rt_get_launch_config(const rt_ctx *ctx, rt_launch_config *cfg);
+
+/* ---- graphs of repeated launches ------------------------------------------ */
+
+/* Record every rt_render_device call made between rt_capture_begin and
+ * rt_capture_end into one HIP graph (no other rt_* call that allocates,
+ * copies or synchronises is allowed in between).  rt_graph_launch replays it
+ * on the context's stream without synchronising; rt_last_kernel_ms then
+ * times the whole replay.  No counterpart in the reference, whose pixel loop
+ * is one blocking call (core.c:97-114). */
+typedef struct rt_graph rt_graph;
+int rt_capture_begin(rt_ctx *ctx);
+int rt_capture_end(rt_ctx *ctx, rt_graph **graph);
+int rt_graph_launch(rt_ctx *ctx, rt_graph *graph);
+int rt_graph_destroy(rt_ctx *ctx, rt_graph *graph);
 
 /* ---- device buffers for callers that keep outputs in HBM ---------------- */
 int rt_device_alloc(rt_ctx *ctx, size_t bytes, void **d_ptr);

@@ -31,8 +31,10 @@ namespace {
 struct TraceArgs {
     const float4 *prims;
     unsigned int *queue;
-    int total;
+    int total;          // work items: n_cam * (tiles * tile^2, or W*H)
     int refill;
+    int tile, ntx;      // work-item order: T x T pixel tiles, ntx tiles per row
+    int per_cam;        // work items per camera
     double *image;
     double *dimage;
     rt_ray_counters *counters;
@@ -60,6 +62,41 @@ __device__ inline void write_ray(const Params &P, const TraceArgs &A, int ray,
     }
 }
 
+// Work item -> (camera, pixel).  tile <= 1: row-major pixels; otherwise
+// T x T tiles in row-major tile order, pixels row-major inside a tile, and
+// items that fall outside a ragged last tile row/column are skipped.
+__device__ inline bool work_to_ray(const Params &P, const TraceArgs &A, int w, int &ray)
+{
+    const int cam = w / A.per_cam;
+    const int r = w - cam * A.per_cam;
+    int pix;
+    if (A.tile <= 1) {
+        pix = r;
+    } else {
+        const int tt = A.tile * A.tile;
+        const int t = r / tt, l = r - t * tt;
+        const int x = (t % A.ntx) * A.tile + l % A.tile;
+        const int y = (t / A.ntx) * A.tile + l / A.tile;
+        if (x >= P.width || y >= P.height)
+            return false;
+        pix = y * P.width + x;
+    }
+    ray = cam * P.npix + pix;
+    return true;
+}
+
+// Claims work items until one maps to a pixel or the queue is exhausted.
+__device__ inline bool claim(const Params &P, const TraceArgs &A, int &ray)
+{
+    while (true) {
+        const int w = (int)atomicAdd(A.queue, 1u);
+        if (w >= A.total)
+            return false;
+        if (work_to_ray(P, A, w, ray))
+            return true;
+    }
+}
+
 // Each lane traces one ray at a time.  A lane whose ray has ended waits until
 // at least `refill` lanes of its wave are idle; then all idle lanes claim new
 // ray ids together.  Lanes claiming together get consecutive ids (the atomic
@@ -71,9 +108,8 @@ template <int NF, bool GRAD, bool DIM3>
 __global__ __launch_bounds__(64) void trace_kernel(const Params P, const TraceArgs A)
 {
     Ray<NF, GRAD> R;
-    const int total = A.total;
-    int ray = (int)atomicAdd(A.queue, 1u);
-    bool active = ray < total;
+    int ray = 0;
+    bool active = claim(P, A, ray);
     bool exhausted = !active;
     if (active)
         ray_init<NF, GRAD>(P, ray / P.npix, ray % P.npix, R);
@@ -95,8 +131,7 @@ __global__ __launch_bounds__(64) void trace_kernel(const Params P, const TraceAr
                 break;
         } else if (idle >= A.refill || (act == 0ull)) {
             if (!active) {
-                ray = (int)atomicAdd(A.queue, 1u);
-                if (ray < total) {
+                if (claim(P, A, ray)) {
                     ray_init<NF, GRAD>(P, ray / P.npix, ray % P.npix, R);
                     active = true;
                 } else {
@@ -243,8 +278,15 @@ struct rt_ctx {
     double *d_image = nullptr, *d_dimage = nullptr;
     size_t image_cap = 0, dimage_cap = 0;
     double last_ms = 0.;
-    rt_launch_config cfg{64, 0, kDefaultRefill};
+    rt_launch_config cfg{64, 0, kDefaultRefill, 1};
+    bool capturing = false;   // between rt_capture_begin and rt_capture_end
     char err[512] = {0};
+};
+
+struct rt_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int device = 0;
 };
 
 static char g_create_err[512];
@@ -359,14 +401,27 @@ static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *fr
         return rc;
     A.prims = c->d_prims;
     A.queue = c->d_queue;
-    A.total = ncam * P.npix;
     A.refill = std::min(64, std::max(1, c->cfg.refill));
+    A.tile = std::max(1, c->cfg.tile);
+    if (A.tile > 1) {
+        A.ntx = (P.width + A.tile - 1) / A.tile;
+        int nty = (P.height + A.tile - 1) / A.tile;
+        long long pc = (long long)A.ntx * nty * A.tile * A.tile;
+        if (pc * ncam > 0x7fffff00LL)
+            return fail(c, RT_E_INVALID, "too many work items");
+        A.per_cam = (int)pc;
+    } else {
+        A.ntx = 0;
+        A.per_cam = P.npix;
+    }
+    A.total = ncam * A.per_cam;
     A.image = d_image;
     A.dimage = grad ? d_dimage : nullptr;
     A.counters = d_counters;
     hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if (!c->capturing)
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     hipError_t e;
     bool d3 = c->grid.n3 > 1;
     if (grad)
@@ -374,8 +429,16 @@ static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *fr
     else
         e = d3 ? dispatch_nf<false, true>(c, P, A) : dispatch_nf<false, false>(c, P, A);
     HIPCHK(c, e);
-    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    if (!c->capturing)
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     return RT_OK;
+}
+
+// Calls that allocate, copy or synchronise cannot be recorded into a graph.
+static int no_capture(rt_ctx *c, const char *what)
+{
+    return c->capturing ? fail(c, RT_E_STATE, "%s cannot be called while capturing a graph", what)
+                        : RT_OK;
 }
 
 static int render_host(rt_ctx *c, const double *incl, int ncam, const double *freqs, int nf,
@@ -383,6 +446,8 @@ static int render_host(rt_ctx *c, const double *incl, int ncam, const double *fr
 {
     if (!c)
         return RT_E_INVALID;
+    if (int rc0 = no_capture(c, "rt_render"))
+        return rc0;
     if (!h_image || (grad && !h_dimage))
         return fail(c, RT_E_INVALID, "output buffers must not be NULL");
     if (!c->have_params || !c->have_grid)
@@ -502,6 +567,8 @@ int rt_load_prims(rt_ctx *c, const rt_grid *g, const float *h_prims)
 {
     if (!c)
         return RT_E_INVALID;
+    if (int rc0 = no_capture(c, "rt_load_prims"))
+        return rc0;
     if (!g || !h_prims)
         return fail(c, RT_E_INVALID, "grid and prims must not be NULL");
     if (g->n1 < 2 || g->n2 < 2 || g->n3 < 1)
@@ -568,6 +635,8 @@ int rt_synchronize(rt_ctx *c)
 {
     if (!c)
         return RT_E_INVALID;
+    if (int rc0 = no_capture(c, "rt_synchronize"))
+        return rc0;
     int rc = select_device(c);
     if (rc)
         return rc;
@@ -591,17 +660,97 @@ double rt_jansky_factor(const rt_ctx *c)
     return 1.e23 * (dx / (double)p.width) * (dy / (double)p.height);
 }
 
+int rt_capture_begin(rt_ctx *c)
+{
+    if (!c)
+        return RT_E_INVALID;
+    if (c->capturing)
+        return fail(c, RT_E_STATE, "a capture is already open");
+    int rc = select_device(c);
+    if (rc)
+        return rc;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    return RT_OK;
+}
+
+int rt_capture_end(rt_ctx *c, rt_graph **out)
+{
+    if (!c || !out)
+        return RT_E_INVALID;
+    if (!c->capturing)
+        return fail(c, RT_E_STATE, "no capture is open");
+    c->capturing = false;
+    int rc = select_device(c);
+    if (rc)
+        return rc;
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamEndCapture(c->stream, &g));
+    rt_graph *G = new (std::nothrow) rt_graph();
+    if (!G) {
+        (void)hipGraphDestroy(g);
+        return fail(c, RT_E_NOMEM, "out of host memory");
+    }
+    G->graph = g;
+    G->device = c->device;
+    hipError_t e = hipGraphInstantiate(&G->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        delete G;
+        return fail(c, RT_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    *out = G;
+    return RT_OK;
+}
+
+int rt_graph_launch(rt_ctx *c, rt_graph *g)
+{
+    if (!c || !g)
+        return RT_E_INVALID;
+    if (g->device != c->device)
+        return fail(c, RT_E_INVALID, "graph was captured on another device");
+    if (int rc0 = no_capture(c, "rt_graph_launch"))
+        return rc0;
+    int rc = select_device(c);
+    if (rc)
+        return rc;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, hipGraphLaunch(g->exec, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    return RT_OK;
+}
+
+int rt_graph_destroy(rt_ctx *c, rt_graph *g)
+{
+    if (!c)
+        return RT_E_INVALID;
+    if (!g)
+        return RT_OK;
+    int rc = select_device(c);
+    if (rc)
+        return rc;
+    (void)hipStreamSynchronize(c->stream);
+    if (g->exec)
+        (void)hipGraphExecDestroy(g->exec);
+    if (g->graph)
+        (void)hipGraphDestroy(g->graph);
+    delete g;
+    return RT_OK;
+}
+
 int rt_set_launch_config(rt_ctx *c, const rt_launch_config *cfg)
 {
     if (!c)
         return RT_E_INVALID;
     if (!cfg || (cfg->block != 0 && cfg->block != 64) || cfg->blocks_per_cu < 0 ||
-        cfg->blocks_per_cu > 64 || cfg->refill < 0 || cfg->refill > 64)
+        cfg->blocks_per_cu > 64 || cfg->refill < 0 || cfg->refill > 64 || cfg->tile < 0 ||
+        cfg->tile > 64)
         return fail(c, RT_E_INVALID, "bad launch config (block 64, blocks_per_cu 0..64, "
-                                     "refill 0..64)");
+                                     "refill 0..64, tile 0..64)");
     c->cfg.block = 64;
     c->cfg.blocks_per_cu = cfg->blocks_per_cu;
     c->cfg.refill = cfg->refill == 0 ? kDefaultRefill : cfg->refill;
+    c->cfg.tile = cfg->tile == 0 ? 1 : cfg->tile;
     return RT_OK;
 }
 
@@ -617,6 +766,8 @@ int rt_device_alloc(rt_ctx *c, size_t bytes, void **d_ptr)
 {
     if (!c || !d_ptr)
         return RT_E_INVALID;
+    if (int rc0 = no_capture(c, "rt_device_alloc"))
+        return rc0;
     int rc = select_device(c);
     if (rc)
         return rc;
@@ -628,6 +779,8 @@ int rt_device_free(rt_ctx *c, void *d_ptr)
 {
     if (!c)
         return RT_E_INVALID;
+    if (int rc0 = no_capture(c, "rt_device_free"))
+        return rc0;
     int rc = select_device(c);
     if (rc)
         return rc;
@@ -639,6 +792,8 @@ int rt_memcpy_d2h(rt_ctx *c, void *h_dst, const void *d_src, size_t bytes)
 {
     if (!c || (!h_dst && bytes) || (!d_src && bytes))
         return RT_E_INVALID;
+    if (int rc0 = no_capture(c, "rt_memcpy_d2h"))
+        return rc0;
     int rc = select_device(c);
     if (rc)
         return rc;
@@ -651,6 +806,8 @@ int rt_fluid_params(rt_ctx *c, const double *h_X, int n, double *h_out)
 {
     if (!c)
         return RT_E_INVALID;
+    if (int rc0 = no_capture(c, "rt_fluid_params"))
+        return rc0;
     if (n < 0 || (n && (!h_X || !h_out)))
         return fail(c, RT_E_INVALID, "bad point list");
     if (n == 0)
@@ -694,6 +851,8 @@ int rt_probe(rt_ctx *c, int kind, const double *h_in, int n, double *h_out)
 {
     if (!c)
         return RT_E_INVALID;
+    if (int rc0 = no_capture(c, "rt_probe"))
+        return rc0;
     if (n < 0 || (n && (!h_in || !h_out)))
         return fail(c, RT_E_INVALID, "bad point list");
     int nin, nout;

@@ -44,7 +44,7 @@ class RtParams(ctypes.Structure):
 
 class RtLaunchConfig(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
-                ("refill", ctypes.c_int)]
+                ("refill", ctypes.c_int), ("tile", ctypes.c_int)]
 
 
 class RtError(RuntimeError):
@@ -67,6 +67,10 @@ SIGNATURES = [
     ("rt_jansky_factor", ctypes.c_double, [_vp]),
     ("rt_set_launch_config", ctypes.c_int, [_vp, ctypes.POINTER(RtLaunchConfig)]),
     ("rt_get_launch_config", ctypes.c_int, [_vp, ctypes.POINTER(RtLaunchConfig)]),
+    ("rt_capture_begin", ctypes.c_int, [_vp]),
+    ("rt_capture_end", ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    ("rt_graph_launch", ctypes.c_int, [_vp, _vp]),
+    ("rt_graph_destroy", ctypes.c_int, [_vp, _vp]),
     ("rt_device_alloc", ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)]),
     ("rt_device_free", ctypes.c_int, [_vp, _vp]),
     ("rt_memcpy_d2h", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
@@ -96,6 +100,21 @@ def load_library(path=LIB_PATH):
 
 def _d(a):
     return a.ctypes.data_as(_dp)
+
+
+class Graph:
+    """A captured sequence of render_device calls (rt_capture_begin/end)."""
+
+    def __init__(self, tracer, handle):
+        self.tracer, self.handle = tracer, handle
+
+    def launch(self):
+        self.tracer._check(self.tracer.lib.rt_graph_launch(self.tracer.ctx, self.handle))
+
+    def destroy(self):
+        if self.handle:
+            self.tracer._check(self.tracer.lib.rt_graph_destroy(self.tracer.ctx, self.handle))
+            self.handle = None
 
 
 class Tracer:
@@ -155,14 +174,15 @@ class Tracer:
         self._check(self.lib.rt_load_prims(self.ctx, ctypes.byref(grid),
                                            prims.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
 
-    def set_launch_config(self, blocks_per_cu=0, refill=0):
-        cfg = RtLaunchConfig(block=64, blocks_per_cu=blocks_per_cu, refill=refill)
+    def set_launch_config(self, blocks_per_cu=0, refill=0, tile=0):
+        cfg = RtLaunchConfig(block=64, blocks_per_cu=blocks_per_cu, refill=refill, tile=tile)
         self._check(self.lib.rt_set_launch_config(self.ctx, ctypes.byref(cfg)))
 
     def get_launch_config(self):
         cfg = RtLaunchConfig()
         self._check(self.lib.rt_get_launch_config(self.ctx, ctypes.byref(cfg)))
-        return dict(block=cfg.block, blocks_per_cu=cfg.blocks_per_cu, refill=cfg.refill)
+        return dict(block=cfg.block, blocks_per_cu=cfg.blocks_per_cu, refill=cfg.refill,
+                    tile=cfg.tile)
 
     # -- the pixel loop -----------------------------------------------------------
     def render(self, inclinations, freqs, grad=False):
@@ -196,6 +216,15 @@ class Tracer:
 
     def jansky_factor(self):
         return self.lib.rt_jansky_factor(self.ctx)
+
+    # -- graphs ------------------------------------------------------------------
+    def capture_begin(self):
+        self._check(self.lib.rt_capture_begin(self.ctx))
+
+    def capture_end(self):
+        g = _vp()
+        self._check(self.lib.rt_capture_end(self.ctx, ctypes.byref(g)))
+        return Graph(self, g.value)
 
     # -- device buffers -----------------------------------------------------------
     def alloc(self, nbytes):

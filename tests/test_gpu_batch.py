@@ -1,0 +1,125 @@
+"""Batched launches, launch-shape knobs, graph replay and the autotuner.
+
+A ray's arithmetic does not depend on which lane, wave or launch traces it,
+so every launch shape must give bit-identical images, and a batched launch
+over cameras x frequencies must equal the per-image launches."""
+import numpy as np
+import pytest
+
+from conftest import FREQ, INCL, M_UNIT, R_HIGH
+from raptor_amd import autotune, rt
+
+pytestmark = pytest.mark.gpu
+
+C2_INCL = [10.0, 25.0, 40.0, 55.0, 70.0, 85.0]
+C2_FREQ = list(86e9 * (690.0 / 86.0) ** (np.arange(8) / 7.0))
+
+
+@pytest.fixture(scope="module")
+def tracer(dump128):
+    tr = rt.Tracer(0)
+    tr.load_dump(dump128)
+    yield tr
+    tr.close()
+
+
+def _params(tr, W, H=None):
+    tr.set_params(width=W, height=W if H is None else H, M_unit=M_UNIT, R_high=R_HIGH)
+
+
+@pytest.mark.parametrize("cfg", [dict(tile=4), dict(tile=8), dict(tile=16),
+                                 dict(refill=64), dict(refill=17, tile=5),
+                                 dict(blocks_per_cu=1), dict(blocks_per_cu=12, tile=3)])
+def test_launch_shapes_are_bit_identical(tracer, cfg):
+    _params(tracer, 45, 37)                      # ragged against every tile size
+    tracer.set_launch_config()
+    ref, _ = tracer.render([INCL], [FREQ])
+    tracer.set_launch_config(**cfg)
+    try:
+        got, _ = tracer.render([INCL], [FREQ])
+    finally:
+        tracer.set_launch_config()
+    assert np.array_equal(got, ref)
+
+
+def test_bad_launch_config_rejected(tracer):
+    with pytest.raises(rt.RtError):
+        tracer.set_launch_config(refill=65)
+    with pytest.raises(rt.RtError):
+        tracer.set_launch_config(tile=-1)
+
+
+def test_c2_batched_equals_per_image(tracer):
+    """Six inclinations x eight log-spaced frequencies (86-690 GHz) in one
+    launch equal 48 single-image launches."""
+    _params(tracer, 32)
+    batched, spec = tracer.render(C2_INCL, C2_FREQ)
+    for c, inc in enumerate(C2_INCL):
+        for f, nu in enumerate(C2_FREQ):
+            one, s1 = tracer.render([inc], [nu])
+            np.testing.assert_array_equal(batched[c, :, f], one[0, :, 0])
+            assert spec[c, f] == s1[0, 0]
+
+
+def test_graph_replay_matches_eager(tracer):
+    """A captured graph of two renders, replayed several times with eager
+    work in between; every replay equals the eager result."""
+    W = 40
+    _params(tracer, W)
+    n = W * W
+    d1 = tracer.alloc(n * 8)
+    d2 = tracer.alloc(n * 3 * 8)
+    dx = tracer.alloc(n * 8)
+    try:
+        tracer.render_device([INCL], [FREQ], d1)
+        tracer.render_device([30.0], C2_FREQ[:3], d2)
+        tracer.synchronize()
+        e1 = tracer.d2h(d1, (n,), np.float64)
+        e2 = tracer.d2h(d2, (n, 3), np.float64)
+        tracer.capture_begin()
+        tracer.render_device([INCL], [FREQ], d1)
+        tracer.render_device([30.0], C2_FREQ[:3], d2)
+        g = tracer.capture_end()
+        try:
+            for k in range(4):
+                # eager work that touches the same queue counter and stream
+                tracer.render_device([80.0], [FREQ], dx)
+                tracer.synchronize()
+                tracer.d2h(dx, (n,), np.float64)
+                g.launch()
+                tracer.synchronize()
+                assert np.array_equal(tracer.d2h(d1, (n,), np.float64), e1), k
+                assert np.array_equal(tracer.d2h(d2, (n, 3), np.float64), e2), k
+        finally:
+            g.destroy()
+    finally:
+        for d in (d1, d2, dx):
+            tracer.free(d)
+
+
+def test_calls_that_synchronise_are_refused_during_capture(tracer):
+    _params(tracer, 8)
+    tracer.capture_begin()
+    try:
+        with pytest.raises(rt.RtError, match="capturing"):
+            tracer.render([INCL], [FREQ])
+        with pytest.raises(rt.RtError, match="capturing"):
+            tracer.synchronize()
+    finally:
+        tracer.capture_end().destroy()
+
+
+def test_autotune_picks_a_valid_config_and_caches_it(tracer, tmp_path):
+    _params(tracer, 48)
+    path = str(tmp_path / "tune.json")
+    res = autotune.tune(tracer, [INCL], [FREQ], space=dict(blocks_per_cu=(0, 4),
+                                                           refill=(1,), tile=(1, 8)),
+                        reps=1, path=path)
+    assert len(res["table"]) == 4 and res["ms"] > 0
+    assert tracer.get_launch_config()["tile"] == res["config"]["tile"]
+    tracer.set_launch_config()
+    assert autotune.apply_cached(tracer, 1, 1, path=path) == res["config"]
+    img, _ = tracer.render([INCL], [FREQ])
+    tracer.set_launch_config()
+    ref, _ = tracer.render([INCL], [FREQ])
+    assert np.array_equal(img, ref)
