@@ -58,6 +58,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=65536,
                     help="pixels of the workload traced by the CPU oracle (0 = skip)")
     ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4"], default="c1",
+                    help="c1: the headline 1024^2 image (default); c2: 6 inclinations x 8 "
+                         "frequencies at 512^2, one launch vs 48; c3: 3D 192x96x96 dump at "
+                         "1024^2; c4: image+gradient on the 3D dump at 512^2")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -66,6 +70,11 @@ def main():
         sys.exit(2)
 
     from raptor_amd import flops, rt, torus
+
+    if args.workload == "c2":
+        return bench_c2(args)
+    if args.workload in ("c3", "c4"):
+        return bench_3d(args)
 
     W = args.size
     with warnings.catch_warnings():
@@ -205,6 +214,146 @@ def main():
     tr.free(d_cnt)
     tr.close()
     print(json.dumps(result), flush=True)
+
+
+def _timed(tr, fn, steps):
+    """Mean kernel ms (HIP events) and wall ms per call of fn()."""
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+        tr.synchronize()
+        kms.append(tr.last_kernel_ms())
+    return float(np.mean(kms)), 1e3 * (time.perf_counter() - t0) / steps
+
+
+def bench_c2(args):
+    """Batched: 6 inclinations x 8 log-spaced frequencies (86-690 GHz) at 512^2
+    in one persistent launch (autotuned shape) against the same 48 images as
+    single-image launches replayed from one HIP graph."""
+    from raptor_amd import autotune, rt, torus
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        dump = torus.fishbone_moncrief(256, 256)
+    W = 512
+    incl = [10.0, 25.0, 40.0, 55.0, 70.0, 85.0]
+    freqs = list(86e9 * (690.0 / 86.0) ** (np.arange(8) / 7.0))
+    tr = rt.Tracer(0)
+    tr.set_params(width=W, height=W, M_unit=WORKLOAD["M_unit"], R_high=WORKLOAD["R_high"],
+                  R_low=WORKLOAD["R_low"])
+    tr.load_dump(dump)
+    npix = W * W
+    tune = autotune.tune(tr, incl, freqs, reps=1, path=None)
+    d_b = tr.alloc(len(incl) * npix * len(freqs) * 8)
+    d_s = tr.alloc(len(incl) * len(freqs) * npix * 8)
+    for _ in range(max(1, args.warmup)):
+        tr.render_device(incl, freqs, d_b)
+    tr.synchronize()
+    kb, wb = _timed(tr, lambda: tr.render_device(incl, freqs, d_b), args.steps)
+    # 48 single-image launches, captured once and replayed
+    tr.set_launch_config()
+    tr.capture_begin()
+    k = 0
+    for inc in incl:
+        for nu in freqs:
+            tr.render_device([inc], [nu], d_s + k * npix * 8)
+            k += 1
+    g = tr.capture_end()
+    g.launch()
+    tr.synchronize()
+    ks, ws = _timed(tr, g.launch, args.steps)
+    b = tr.d2h(d_b, (len(incl), npix, len(freqs)), np.float64)
+    s = tr.d2h(d_s, (len(incl), len(freqs), npix), np.float64)
+    same = bool(np.array_equal(b, np.transpose(s, (0, 2, 1))))
+    g.destroy()
+    tr.free(d_b)
+    tr.free(d_s)
+    tr.close()
+    rays = len(incl) * npix
+    print(json.dumps({
+        "metric": METRIC, "workload": "c2", "unit": "pixel-geodesics/s",
+        "value": rays / (wb * 1e-3), "ms_per_step": wb, "steps": args.steps,
+        "config": dict(workload="harm2d-torus-256x256_img512x512_6incl_8freq", dump="256x256x1",
+                       image="512x512", inclinations=incl, freqs_hz=freqs),
+        "batched": {"kernel_ms": kb, "wall_ms": wb, "launch_config": tune["config"],
+                    "images_per_s": len(incl) * len(freqs) / (wb * 1e-3)},
+        "per_image_graph": {"launches": len(incl) * len(freqs), "kernel_ms": ks,
+                            "wall_ms": ws,
+                            "images_per_s": len(incl) * len(freqs) / (ws * 1e-3)},
+        "speedup_batched": ws / wb, "batched_equals_per_image": same,
+        "autotune_table": [[c, round(m, 3)] for c, m in tune["table"]],
+    }), flush=True)
+
+
+def bench_3d(args):
+    """c3: the 3D 192x96x96 torus (m=1..4 azimuthal modes, 10%) at 1024^2, 230
+    GHz.  c4: the same dump at 512^2, image + d/d(ln M_unit, R_high) against
+    the image alone, with a central-difference check of the gradient."""
+    from raptor_amd import rt, torus
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        dump = torus.fishbone_moncrief(192, 96, 96)
+    W = 1024 if args.workload == "c3" else 512
+    npix = W * W
+    incl, freq = [WORKLOAD["inclination_deg"]], [WORKLOAD["freq_hz"]]
+    tr = rt.Tracer(0)
+    kw = dict(width=W, height=W, R_low=WORKLOAD["R_low"])
+    tr.set_params(M_unit=WORKLOAD["M_unit"], R_high=WORKLOAD["R_high"], **kw)
+    tr.load_dump(dump)
+    d_img = tr.alloc(npix * 8)
+    d_dimg = tr.alloc(npix * 16)
+    for _ in range(max(1, args.warmup)):
+        tr.render_device(incl, freq, d_img)
+    tr.synchronize()
+    ki, wi = _timed(tr, lambda: tr.render_device(incl, freq, d_img), args.steps)
+    out = {"metric": METRIC, "workload": args.workload, "unit": "pixel-geodesics/s",
+           "value": npix / (wi * 1e-3), "ms_per_step": wi, "steps": args.steps,
+           "config": dict(workload="harm3d-torus-192x96x96_img%dx%d_230GHz_i60" % (W, W),
+                          dump="192x96x96", image="%dx%d" % (W, W),
+                          prims_MB=dump.prims.size * 4 / 1e6),
+           "image": {"kernel_ms": ki, "wall_ms": wi, "images_per_s": 1e3 / wi}}
+    if args.workload == "c4":
+        tr.render_device(incl, freq, d_img, d_dimg)
+        tr.synchronize()
+        kg, wg = _timed(tr, lambda: tr.render_device(incl, freq, d_img, d_dimg), args.steps)
+        an = tr.d2h(d_dimg, (npix, 2), np.float64)
+        # central differences (step 1e-3 and 5e-4) of the FP64 oracle on a
+        # pixel sample; GPU-image differences would be FP32-noise limited
+        from oracle import oracle as O
+        M0, R0 = WORKLOAD["M_unit"], WORKLOAD["R_high"]
+        pix = np.arange(0, npix, max(1, npix // 2048), dtype=np.int32)[:2048]
+
+        def ora(M, R):
+            P = O.params(dump.grid, width=W, height=W, inclination=incl[0], M_unit=M,
+                         R_high=R, R_low=WORKLOAD["R_low"])
+            return O.render(P, dump.prims, freq, pixels=pix, nthreads=host_cores())[:, 0]
+
+        stats = {}
+        for q, name in enumerate(("dlnMunit", "dRhigh")):
+            fds = []
+            for h in (1e-3, 5e-4):
+                if q == 0:
+                    fd = (ora(M0 * np.exp(h), R0) - ora(M0 * np.exp(-h), R0)) / (2 * h)
+                else:
+                    fd = (ora(M0, R0 * (1 + h)) - ora(M0, R0 * (1 - h))) / (2 * h * R0)
+                fds.append(fd)
+            a = an[pix, q]
+            den = np.maximum(np.abs(a), 1e-3 * np.abs(a).max())
+            smooth = np.abs(fds[0] - fds[1]) / den < 1e-3
+            err = np.abs(fds[0] - a) / den
+            stats[name] = {"pixels": int(pix.size), "smooth_frac": float(smooth.mean()),
+                           "max_rel_smooth": float(err[smooth].max()),
+                           "frac_within_1e-3": float((err[smooth] < 1e-3).mean()),
+                           "sum_rel": float(fds[0][smooth].sum() / a[smooth].sum() - 1)}
+        out["grad"] = {"kernel_ms": kg, "wall_ms": wg, "image_plus_grad_per_s": 1e3 / wg,
+                       "ratio_vs_image": wi / wg, "fd_check": stats,
+                       "fd_note": "GPU tangents vs central differences (step 1e-3) of the "
+                                  "FP64 oracle on 2048 sampled pixels; pixels whose quotient "
+                                  "changes between steps 1e-3 and 5e-4 (gate flips) excluded"}
+    tr.free(d_img)
+    tr.free(d_dimg)
+    tr.close()
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":
