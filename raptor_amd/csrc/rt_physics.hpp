@@ -122,6 +122,64 @@ __host__ __device__ inline Point make_point(double X1, double X2, double h)
     return p;
 }
 
+// The RK2 midpoint differs from the step's start point by e1 = dl k^1 / 2 in
+// X1 and e2 = dl k^2 / 2 in X2 (|dl k^i| <= STEPSIZE by the step-size rule),
+// so its trigonometry follows from the start point's by short Taylor series
+// and angle addition.  The truncation errors are below 1e-17 within the
+// guarded ranges; outside them (never on rays traced with the reference's
+// step size) the point is evaluated in full.
+__host__ __device__ inline double exp_small(double x)   // |x| <= 0.05
+{
+    double p = 1. / 40320.;
+    p = p * x + 1. / 5040.;
+    p = p * x + 1. / 720.;
+    p = p * x + 1. / 120.;
+    p = p * x + 1. / 24.;
+    p = p * x + 1. / 6.;
+    p = p * x + 0.5;
+    p = p * x + 1.;
+    return p * x + 1.;
+}
+
+__host__ __device__ inline void sincos_small(double x, double &s, double &c)   // |x| <= 0.1
+{
+    const double x2 = x * x;
+    double ps = -1. / 39916800.;
+    ps = ps * x2 + 1. / 362880.;
+    ps = ps * x2 - 1. / 5040.;
+    ps = ps * x2 + 1. / 120.;
+    ps = ps * x2 - 1. / 6.;
+    s = x + x * x2 * ps;
+    double pc = 1. / 3628800.;
+    pc = pc * x2 - 1. / 40320.;
+    pc = pc * x2 + 1. / 720.;
+    pc = pc * x2 - 1. / 24.;
+    pc = pc * x2 + 0.5;
+    c = 1. - x2 * pc;
+}
+
+__host__ __device__ inline Point shifted_point(const Point &p0, double X1m, double X2m,
+                                               double e1, double e2, double h)
+{
+    const double kh = (1. - h) * (0.5 / kPi);
+    const double a2 = 2. * kPi * e2;
+    if (!(fabs(e1) <= 0.05 && fabs(a2) <= 0.1))
+        return make_point(X1m, X2m, h);
+    Point p;
+    p.r = p0.r * exp_small(e1);
+    double ds, dc;
+    sincos_small(a2, ds, dc);
+    p.s2 = p0.s2 * dc + p0.c2 * ds;
+    p.c2 = p0.c2 * dc - p0.s2 * ds;
+    const double at = kPi * (e2 + kh * (p.s2 - p0.s2));   // theta_m - theta_0
+    if (!(fabs(at) <= 0.1))
+        return make_point(X1m, X2m, h);
+    sincos_small(at, ds, dc);
+    p.s = p0.s * dc + p0.c * ds;
+    p.c = p0.c * dc - p0.s * ds;
+    return p;
+}
+
 // Covariant metric (metric.c:94-138), 10 unique entries:
 // g00 g01 g03 g11 g13 g22 g33 (g02 = g12 = g23 = 0).
 struct MetricDD {
@@ -159,8 +217,7 @@ __host__ __device__ inline void lower(const MetricDD &g, const double v[4], doub
 __host__ __device__ inline void geodesic_accel(const Point &p, double a, double h,
                                                const double k[4], double A[4])
 {
-    const double r = p.r, r2 = r * r;
-    const double rp = p.r, rp2 = rp * rp;
+    const double r = p.r, r2 = r * r;   // r = r' = r'' = exp(X1) for R0 = 0
     const double thp = kPi * (1. + (1. - h) * p.c2);
     const double thp2 = thp * thp;
     const double thpp = -2. * kPi * kPi * (1. - h) * p.s2;
@@ -183,59 +240,46 @@ __host__ __device__ inline void geodesic_accel(const Point &p, double a, double 
     const double B = (2. * r2 - Sig) * iS3;
     const double C = r * iS - a2 * B * sin2;
 
-    // products k^a k^b, off-diagonal ones doubled
-    const double k00 = k[0] * k[0], k11 = k[1] * k[1], k22 = k[2] * k[2], k33 = k[3] * k[3];
-    const double k01 = 2. * k[0] * k[1], k02 = 2. * k[0] * k[2], k03 = 2. * k[0] * k[3];
-    const double k12 = 2. * k[1] * k[2], k13 = 2. * k[1] * k[3], k23 = 2. * k[2] * k[3];
-
-    // Gamma^t
-    const double G000 = 2. * r * B;
-    const double G001 = (Sig + 2. * r) * B * rp;
-    const double G002 = -a2 * r * s2th * iS2 * thp;
-    const double G003 = -2. * a * r * B * sin2;
-    const double G011 = 2. * (Sig + r) * B * rp2;
-    const double G012 = G002 * rp;
-    const double G013 = -a * sin2 * G001;
-    const double G022 = -2. * r2 * iS * thp2;
-    const double G023 = -a * sin2 * G002;
-    const double G033 = -2. * r * C * sin2;
-    A[0] = -(G000 * k00 + G001 * k01 + G002 * k02 + G003 * k03 + G011 * k11 + G012 * k12 +
-             G013 * k13 + G022 * k22 + G023 * k23 + G033 * k33);
-    // Gamma^r (Gamma^1_02 = Gamma^1_23 = 0)
-    const double G100 = Del * B * irp;
-    const double G101 = (Del - Sig) * B;
-    const double G103 = -a * Del * B * sin2 * irp;
-    const double G111 = 1. - (2. * Sig - Del) * B * rp;
+    // The contraction A^m = -Gamma^m_ab k^a k^b, grouped over the entries of
+    // metric.c:568-637 that share factors.  With u = k0 - a sin^2 k3 and
+    // w = k0 + r k1 (r' = r, r'' = r' for R0 = 0):
+    //   Gamma^t: G000 k0^2 + 2 G003 k0 k3 = 2rB k0 (u - a sin^2 k3);
+    //            2 G001 k0 k1 + 2 G013 k1 k3 = 2 G001 k1 u;
+    //            2 (G002 k0 + G012 k1 + G023 k3) k2 = 2 G002 k2 (u + r k1);
+    //   Gamma^r: G100 k0^2 + 2 G103 k0 k3 + G133 k3^2
+    //              = Delta/r (B k0 (u - a sin^2 k3) - C sin^2 k3^2);
+    //            2 (G101 k0 + G113 k3) k1 = 2 k1 ((Delta - Sigma) B u + a r sin^2 k3 / Sigma);
+    //   Gamma^theta: G200 k0^2 + 2 G201 k0 k1 + G211 k1^2 = G200 w^2, and G200, G203,
+    //            G213, G233 share sin cos / (Sigma^3 theta');
+    //   Gamma^phi: G300 k0^2 + 2 G301 k0 k1 + G311 k1^2 = a B w^2;
+    //            2 (G302 k0 + G312 k1) k2 = -2 a cot theta' k2 (2 r k0 + (Sigma + 2r) r k1) / Sigma^2;
+    //            2 G313 k1 k3 + G333 k3^2 = C k3 (2 r k1 - a sin^2 k3).
+    const double k0 = k[0], k1 = k[1], k2 = k[2], k3 = k[3];
+    const double as2 = a * sin2;
+    const double u = k0 - as2 * k3;
+    const double rk1 = r * k1;
+    const double w = k0 + rk1;
+    const double Sp2r = Sig + 2. * r;
+    const double iS2thp = iS2 * thp;
+    const double G002 = -a2 * r * s2th * iS2thp;
     const double G112 = -a2 * s * c * iS * thp;
-    const double G113 = a * (r * iS + (Sig - Del) * B) * sin2;
-    const double G122 = -r * Del * iS * irp * thp2;
-    const double G133 = -Del * C * sin2 * irp;
-    A[1] = -(G100 * k00 + G101 * k01 + G103 * k03 + G111 * k11 + G112 * k12 + G113 * k13 +
-             G122 * k22 + G133 * k33);
-    // Gamma^theta (Gamma^2_02 = Gamma^2_23 = 0)
-    const double G200 = iS * G002 * (ithp * ithp);
-    const double G201 = G200 * rp;
-    const double G203 = a * r * (r2 + a2) * s2th * iS3 * ithp;
-    const double G211 = G200 * rp2;
-    const double G212 = r * iS * rp;
-    const double G213 = a * s * c * (Acap + Sig * (Sig - Del)) * iS3 * rp * ithp;
-    const double G222 = thpp * ithp + G112;
-    const double G233 = -s * c * (Del * Sig * Sig + 2. * r * (r2 + a2) * (r2 + a2)) * iS3 * ithp;
-    A[2] = -(G200 * k00 + G201 * k01 + G203 * k03 + G211 * k11 + G212 * k12 + G213 * k13 +
-             G222 * k22 + G233 * k33);
-    // Gamma^phi
-    const double G300 = a * B;
-    const double G301 = G300 * rp;
-    const double G302 = -2. * a * r * cot * iS2 * thp;
-    const double G303 = -a2 * B * sin2;
-    const double G311 = G300 * rp2;
-    const double G312 = -a * (Sig + 2. * r) * cot * iS2 * rp * thp;
-    const double G313 = C * rp;
-    const double G322 = -a * r * iS * thp2;
-    const double G323 = (cot + a2 * r * s2th * iS2) * thp;
-    const double G333 = -a * C * sin2;
-    A[3] = -(G300 * k00 + G301 * k01 + G302 * k02 + G303 * k03 + G311 * k11 + G312 * k12 +
-             G313 * k13 + G322 * k22 + G323 * k23 + G333 * k33);
+    const double k22 = k2 * k2, k33 = k3 * k3;
+    A[0] = -(2. * r * B * (k0 * (u - as2 * k3) + k1 * (Sp2r * u + (Sig + r) * rk1)) +
+             2. * G002 * k2 * (u + rk1) - 2. * r2 * iS * thp2 * k22 - 2. * r * C * sin2 * k33);
+    A[1] = -(Del * irp * (B * k0 * (u - as2 * k3) - C * sin2 * k33) +
+             2. * k1 * ((Del - Sig) * B * u + a * r * iS * sin2 * k3) +
+             (1. - (2. * Sig - Del) * B * r) * k1 * k1 + 2. * G112 * k1 * k2 -
+             Del * iS * thp2 * k22);
+    const double ra2 = r2 + a2;
+    const double Q = s * c * iS3 * ithp;
+    A[2] = -(Q * (-2. * a2 * r * w * w + 4. * a * r * ra2 * k0 * k3 +
+                  2. * a * r * (Acap + Sig * (Sig - Del)) * k1 * k3 -
+                  (Del * Sig * Sig + 2. * r * ra2 * ra2) * k33) +
+             2. * r2 * iS * k1 * k2 + (thpp * ithp + G112) * k22);
+    A[3] = -(a * B * (w * w - 2. * as2 * k0 * k3) -
+             2. * a * cot * iS2thp * k2 * (2. * r * k0 + Sp2r * rk1) +
+             C * k3 * (2. * rk1 - as2 * k3) - a * r * iS * thp2 * k22 +
+             2. * (cot + a2 * r * s2th * iS2) * thp * k2 * k3);
 }
 
 // integrator.c:109-122: dl = -max(1 / sum_i 1/dl_i, 1e-12) with
@@ -264,7 +308,7 @@ __host__ __device__ inline void rk2_step(double y[8], Point &pt, double dl, doub
         ym[i] = y[i] + dl * y[4 + i] * 0.5;
     for (int i = 0; i < 4; i++)
         ym[4 + i] = y[4 + i] + dl * A[i] * 0.5;
-    Point pm = make_point(ym[1], ym[2], h);
+    Point pm = shifted_point(pt, ym[1], ym[2], ym[1] - y[1], ym[2] - y[2], h);
     geodesic_accel(pm, a, h, ym + 4, A);
     for (int i = 1; i < 4; i++)
         y[i] = y[i] + dl * ym[4 + i];

@@ -81,3 +81,18 @@ extern "C" int hc_render(const rt_params *p, const rt_grid *g, const float *soa,
     delete P;
     return 0;
 }
+
+// The midpoint trigonometry from the start point (shifted_point) next to a
+// full evaluation at the midpoint: out[0..4] shifted, out[5..9] full.
+extern "C" void hc_shifted_point(double X1, double X2, double e1, double e2, double h,
+                                 double *out)
+{
+    Point p0 = make_point(X1, X2, h);
+    Point a = shifted_point(p0, X1 + e1, X2 + e2, e1, e2, h);
+    Point b = make_point(X1 + e1, X2 + e2, h);
+    const double va[5] = {a.r, a.s, a.c, a.s2, a.c2}, vb[5] = {b.r, b.s, b.c, b.s2, b.c2};
+    for (int i = 0; i < 5; i++) {
+        out[i] = va[i];
+        out[5 + i] = vb[i];
+    }
+}
