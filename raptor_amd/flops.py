@@ -13,11 +13,16 @@ division, square/cube root, exp, sin or cos counts 1; products of literal
 constants are folded.  Per-function breakdown:
 
   make_point            5 arith + 5 transcendental (exp, 2 sincospi) = 10
-  geodesic_accel      249 arith + 1 division                         = 250
+  geodesic_accel      197 arith + 1 division                         = 198
+    (45 shared prelude: Sigma, Delta, the reciprocals, B, C; 17 shared
+    contraction terms u, w, G002, G112; 27 / 34 / 40 / 35 for A^t, A^r,
+    A^theta, A^phi)
   stepsize              3 arith + 2 division                         = 5
-  rk2_step            2*accel + 2*make_point + 40 updates            = 560
+  rk2_step            2*accel + 2*make_point + 37 updates            = 453
+    (the midpoint counts as a full make_point: its evaluation by angle
+    addition from the start point is an implementation choice)
   loop bookkeeping      1                                             = 1
-  F_STEP = 5 + 560 + 1                                                = 566
+  F_STEP = 5 + 453 + 1                                                = 459
     of which 10 transcendental, 4 division
 
   fluid_sample (FP32) 231 arith + 9 reciprocal + 2 sqrt              = 242 (F_FLUID)
@@ -25,7 +30,7 @@ constants are folded.  Per-function breakdown:
   tangent pass: +11 per sample (dTheta_e/dR_high), +41 per sample-frequency
 """
 
-F_STEP = 566
+F_STEP = 459
 F_STEP_TRANS = 10
 F_STEP_DIV = 4
 F_FLUID = 242
