@@ -137,12 +137,20 @@ double rt_jansky_factor(const rt_ctx *ctx);
  * workgroups per CU (0 = the occupancy limit).  refill: a lane whose ray has
  * ended claims a new one only when at least `refill` lanes of its wave are
  * idle (1..64; 0 = library default).  tile: rays are handed out in T x T
- * pixel tiles (1 or 0 = row-major pixel order). */
+ * pixel tiles (1 = single pixels; 0 = library default, 8).  order: the order
+ * of the tiles, RT_ORDER_CENTRE (by distance from the image centre, so the
+ * long rays near the photon ring start first and the short escaping rays
+ * fill the end of the launch) or RT_ORDER_SCAN (row-major); 0 = library
+ * default (centre).  None of these changes any output bit.  The centre order
+ * for a new image shape is built by the first eager render of that shape. */
+#define RT_ORDER_CENTRE 1
+#define RT_ORDER_SCAN 2
 typedef struct {
     int block;
     int blocks_per_cu;
     int refill;
     int tile;
+    int order;
 } rt_launch_config;
 
 int rt_set_launch_config(rt_ctx *ctx, const rt_launch_config *cfg);

@@ -25,6 +25,7 @@
 using namespace rt;
 
 static constexpr int kDefaultRefill = 1;
+static constexpr int kDefaultTile = 8;   // one wave's 64 lanes take one 8 x 8 tile
 
 namespace {
 
@@ -35,6 +36,7 @@ struct TraceArgs {
     int refill;
     int tile, ntx;      // work-item order: T x T pixel tiles, ntx tiles per row
     int per_cam;        // work items per camera
+    const int *order;   // work item -> pixel within a camera, or NULL (scan)
     double *image;
     double *dimage;
     rt_ray_counters *counters;
@@ -62,15 +64,18 @@ __device__ inline void write_ray(const Params &P, const TraceArgs &A, int ray,
     }
 }
 
-// Work item -> (camera, pixel).  tile <= 1: row-major pixels; otherwise
-// T x T tiles in row-major tile order, pixels row-major inside a tile, and
-// items that fall outside a ragged last tile row/column are skipped.
+// Work item -> (camera, pixel).  With an order table (per_cam = W*H) the
+// table gives the pixel.  Otherwise tile <= 1: row-major pixels; or T x T
+// tiles in row-major tile order, pixels row-major inside a tile, and items
+// that fall outside a ragged last tile row/column are skipped.
 __device__ inline bool work_to_ray(const Params &P, const TraceArgs &A, int w, int &ray)
 {
     const int cam = w / A.per_cam;
     const int r = w - cam * A.per_cam;
     int pix;
-    if (A.tile <= 1) {
+    if (A.order) {
+        pix = A.order[r];
+    } else if (A.tile <= 1) {
         pix = r;
     } else {
         const int tt = A.tile * A.tile;
@@ -275,10 +280,18 @@ struct rt_ctx {
     rt_grid grid{};
     float4 *d_prims = nullptr;
     unsigned int *d_queue = nullptr;
+    // Centre-out claim orders, one per image shape (W, H, tile, pixel size
+    // bits).  A table is never changed or freed before rt_destroy, because
+    // captured graphs keep its address.
+    struct OrderTable {
+        long long key[5];
+        int *d;
+    };
+    std::vector<OrderTable> orders;
     double *d_image = nullptr, *d_dimage = nullptr;
     size_t image_cap = 0, dimage_cap = 0;
     double last_ms = 0.;
-    rt_launch_config cfg{64, 0, kDefaultRefill, 1};
+    rt_launch_config cfg{64, 0, kDefaultRefill, kDefaultTile, RT_ORDER_CENTRE};
     bool capturing = false;   // between rt_capture_begin and rt_capture_end
     char err[512] = {0};
 };
@@ -385,6 +398,82 @@ static int ensure_buffer(rt_ctx *c, double **buf, size_t *cap, size_t bytes)
     return RT_OK;
 }
 
+// Centre-out claim order: the T x T tiles sorted by the distance of their
+// centre from the image centre (in camera units, so non-square pixels are
+// measured correctly), ties in row-major tile order, pixels row-major inside
+// a tile.  Rays near the centre -- the shadow and the photon ring, up to ten
+// times longer than escaping rays -- are claimed first, so the launch does
+// not end waiting on a few long rays that started late.  Built on the host
+// once per image size and tile, kept in HBM (4 bytes per pixel).
+static int ensure_order(rt_ctx *c, int W, int H, int T, double sx, double sy, const int **out)
+{
+    long long bx, by;
+    memcpy(&bx, &sx, sizeof bx);
+    memcpy(&by, &sy, sizeof by);
+    const long long key[5] = {W, H, T, bx, by};
+    for (const auto &t : c->orders) {
+        if (memcmp(key, t.key, sizeof key) == 0) {
+            *out = t.d;
+            return RT_OK;
+        }
+    }
+    if (c->capturing)
+        return fail(c, RT_E_STATE, "the claim order for a new image shape is built by an eager "
+                                   "render; render once before capturing");
+    struct Item {
+        double d;
+        int tile, pix;
+    };
+    const size_t n = (size_t)W * H;
+    std::vector<Item> items;
+    std::vector<int> order;
+    try {
+        items.resize(n);
+        order.resize(n);
+        c->orders.reserve(c->orders.size() + 1);
+    } catch (const std::bad_alloc &) {
+        return fail(c, RT_E_NOMEM, "out of host memory for the claim order");
+    }
+    const int ntx = (W + T - 1) / T;
+    for (int y = 0; y < H; y++) {
+        const int ty = y / T;
+        const double dy = (double)(ty * T + std::min(ty * T + T, H) - H) * sy;  // 2 (centre offset)
+        for (int x = 0; x < W; x++) {
+            const int tx = x / T;
+            const double dx = (double)(tx * T + std::min(tx * T + T, W) - W) * sx;
+            Item &e = items[(size_t)y * W + x];
+            e.d = dx * dx + dy * dy;
+            e.tile = ty * ntx + tx;
+            e.pix = y * W + x;
+        }
+    }
+    std::sort(items.begin(), items.end(), [](const Item &a, const Item &b) {
+        if (a.d != b.d)
+            return a.d < b.d;
+        if (a.tile != b.tile)
+            return a.tile < b.tile;
+        return a.pix < b.pix;
+    });
+    for (size_t i = 0; i < n; i++)
+        order[i] = items[i].pix;
+    rt_ctx::OrderTable t;
+    memcpy(t.key, key, sizeof key);
+    t.d = nullptr;
+    HIPCHK(c, hipMalloc((void **)&t.d, n * sizeof(int)));
+    // on the context's stream, complete before `order` goes out of scope
+    hipError_t e = hipMemcpyAsync(t.d, order.data(), n * sizeof(int), hipMemcpyHostToDevice,
+                                  c->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(t.d);
+        return fail(c, RT_E_HIP, "claim order upload: %s", hipGetErrorString(e));
+    }
+    c->orders.push_back(t);
+    *out = t.d;
+    return RT_OK;
+}
+
 static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *freqs, int nf,
                        double *d_image, double *d_dimage, rt_ray_counters *d_counters, bool grad)
 {
@@ -403,7 +492,15 @@ static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *fr
     A.queue = c->d_queue;
     A.refill = std::min(64, std::max(1, c->cfg.refill));
     A.tile = std::max(1, c->cfg.tile);
-    if (A.tile > 1) {
+    A.order = nullptr;
+    if (c->cfg.order != RT_ORDER_SCAN) {
+        rc = ensure_order(c, P.width, P.height, A.tile, P.cam_x / P.width, P.cam_y / P.height,
+                          &A.order);
+        if (rc)
+            return rc;
+        A.ntx = 0;
+        A.per_cam = P.npix;
+    } else if (A.tile > 1) {
         A.ntx = (P.width + A.tile - 1) / A.tile;
         int nty = (P.height + A.tile - 1) / A.tile;
         long long pc = (long long)A.ntx * nty * A.tile * A.tile;
@@ -534,6 +631,8 @@ void rt_destroy(rt_ctx *c)
         (void)hipStreamSynchronize(c->stream);
     (void)hipFree(c->d_prims);
     (void)hipFree(c->d_queue);
+    for (auto &t : c->orders)
+        (void)hipFree(t.d);
     (void)hipFree(c->d_image);
     (void)hipFree(c->d_dimage);
     if (c->ev0)
@@ -744,13 +843,16 @@ int rt_set_launch_config(rt_ctx *c, const rt_launch_config *cfg)
         return RT_E_INVALID;
     if (!cfg || (cfg->block != 0 && cfg->block != 64) || cfg->blocks_per_cu < 0 ||
         cfg->blocks_per_cu > 64 || cfg->refill < 0 || cfg->refill > 64 || cfg->tile < 0 ||
-        cfg->tile > 64)
+        cfg->tile > 64 || (cfg->order != 0 && cfg->order != RT_ORDER_CENTRE &&
+                           cfg->order != RT_ORDER_SCAN))
         return fail(c, RT_E_INVALID, "bad launch config (block 64, blocks_per_cu 0..64, "
-                                     "refill 0..64, tile 0..64)");
+                                     "refill 0..64, tile 0..64, order 0, RT_ORDER_CENTRE or "
+                                     "RT_ORDER_SCAN)");
     c->cfg.block = 64;
     c->cfg.blocks_per_cu = cfg->blocks_per_cu;
     c->cfg.refill = cfg->refill == 0 ? kDefaultRefill : cfg->refill;
-    c->cfg.tile = cfg->tile == 0 ? 1 : cfg->tile;
+    c->cfg.tile = cfg->tile == 0 ? kDefaultTile : cfg->tile;
+    c->cfg.order = cfg->order == 0 ? RT_ORDER_CENTRE : cfg->order;
     return RT_OK;
 }
 

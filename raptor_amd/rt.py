@@ -44,7 +44,12 @@ class RtParams(ctypes.Structure):
 
 class RtLaunchConfig(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
-                ("refill", ctypes.c_int), ("tile", ctypes.c_int)]
+                ("refill", ctypes.c_int), ("tile", ctypes.c_int), ("order", ctypes.c_int)]
+
+
+# claim orders of rt_launch_config.order (include/raptor_rt.h)
+ORDER_CENTRE = 1
+ORDER_SCAN = 2
 
 
 class RtError(RuntimeError):
@@ -174,15 +179,17 @@ class Tracer:
         self._check(self.lib.rt_load_prims(self.ctx, ctypes.byref(grid),
                                            prims.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
 
-    def set_launch_config(self, blocks_per_cu=0, refill=0, tile=0):
-        cfg = RtLaunchConfig(block=64, blocks_per_cu=blocks_per_cu, refill=refill, tile=tile)
+    def set_launch_config(self, blocks_per_cu=0, refill=0, tile=0, order=0):
+        """order: ORDER_CENTRE, ORDER_SCAN or 0 (library default, centre)."""
+        cfg = RtLaunchConfig(block=64, blocks_per_cu=blocks_per_cu, refill=refill, tile=tile,
+                             order=order)
         self._check(self.lib.rt_set_launch_config(self.ctx, ctypes.byref(cfg)))
 
     def get_launch_config(self):
         cfg = RtLaunchConfig()
         self._check(self.lib.rt_get_launch_config(self.ctx, ctypes.byref(cfg)))
         return dict(block=cfg.block, blocks_per_cu=cfg.blocks_per_cu, refill=cfg.refill,
-                    tile=cfg.tile)
+                    tile=cfg.tile, order=cfg.order)
 
     # -- the pixel loop -----------------------------------------------------------
     def render(self, inclinations, freqs, grad=False):

@@ -29,7 +29,11 @@ def _params(tr, W, H=None):
 
 @pytest.mark.parametrize("cfg", [dict(tile=4), dict(tile=8), dict(tile=16),
                                  dict(refill=64), dict(refill=17, tile=5),
-                                 dict(blocks_per_cu=1), dict(blocks_per_cu=12, tile=3)])
+                                 dict(blocks_per_cu=1), dict(blocks_per_cu=12, tile=3),
+                                 dict(tile=1), dict(order=rt.ORDER_SCAN, tile=1),
+                                 dict(order=rt.ORDER_SCAN), dict(order=rt.ORDER_SCAN, tile=8),
+                                 dict(order=rt.ORDER_SCAN, tile=5, refill=9),
+                                 dict(order=rt.ORDER_CENTRE, tile=64)])
 def test_launch_shapes_are_bit_identical(tracer, cfg):
     _params(tracer, 45, 37)                      # ragged against every tile size
     tracer.set_launch_config()
@@ -47,6 +51,36 @@ def test_bad_launch_config_rejected(tracer):
         tracer.set_launch_config(refill=65)
     with pytest.raises(rt.RtError):
         tracer.set_launch_config(tile=-1)
+    with pytest.raises(rt.RtError):
+        tracer.set_launch_config(order=3)
+    assert tracer.get_launch_config()["order"] == rt.ORDER_CENTRE
+
+
+def test_centre_order_covers_every_pixel_once(tracer):
+    """Per-ray counters in centre order equal those in scan order on ragged
+    shapes (a pixel never traced would keep what the buffer held before: the
+    shapes alternate so that this is another image's data)."""
+    shapes = [(45, 37, 1), (7, 64, 5), (45, 37, 8), (64, 7, 3), (1, 1, 1)]
+    nmax = max(W * H for W, H, _ in shapes)
+    d = tracer.alloc(nmax * 8)
+    dc_scan = tracer.alloc(nmax * 8)
+    dc_centre = tracer.alloc(nmax * 8)
+    try:
+        for W, H, tile in shapes:
+            _params(tracer, W, H)
+            n = W * H
+            got = []
+            for order, dc in ((rt.ORDER_SCAN, dc_scan), (rt.ORDER_CENTRE, dc_centre)):
+                tracer.set_launch_config(tile=tile, order=order)
+                tracer.render_device([INCL], [FREQ], d, None, dc)
+                tracer.synchronize()
+                got.append(tracer.d2h(dc, (n, 2), np.int32))
+            assert (got[0][:, 0] > 0).all(), (W, H, tile)
+            assert np.array_equal(got[0], got[1]), (W, H, tile)
+    finally:
+        tracer.set_launch_config()
+        for p in (d, dc_scan, dc_centre):
+            tracer.free(p)
 
 
 def test_c2_batched_equals_per_image(tracer):
@@ -97,6 +131,41 @@ def test_graph_replay_matches_eager(tracer):
             tracer.free(d)
 
 
+def test_graph_keeps_its_claim_order_after_other_shapes(tracer):
+    """A graph captured at one image shape replays correctly after renders at
+    other shapes have built new claim-order tables; a shape first seen during
+    a capture is refused."""
+    W = 24
+    _params(tracer, W)
+    n = W * W
+    d1 = tracer.alloc(n * 8)
+    try:
+        tracer.render_device([INCL], [FREQ], d1)
+        tracer.synchronize()
+        e1 = tracer.d2h(d1, (n,), np.float64)
+        tracer.capture_begin()
+        tracer.render_device([INCL], [FREQ], d1)
+        g = tracer.capture_end()
+        try:
+            for W2 in (31, 64):
+                _params(tracer, W2)
+                tracer.render([INCL], [FREQ])
+                g.launch()
+                tracer.synchronize()
+                assert np.array_equal(tracer.d2h(d1, (n,), np.float64), e1), W2
+            _params(tracer, 29)
+            tracer.capture_begin()
+            try:
+                with pytest.raises(rt.RtError, match="eager"):
+                    tracer.render_device([INCL], [FREQ], d1)
+            finally:
+                tracer.capture_end().destroy()
+        finally:
+            g.destroy()
+    finally:
+        tracer.free(d1)
+
+
 def test_calls_that_synchronise_are_refused_during_capture(tracer):
     _params(tracer, 8)
     tracer.capture_begin()
@@ -113,7 +182,8 @@ def test_autotune_picks_a_valid_config_and_caches_it(tracer, tmp_path):
     _params(tracer, 48)
     path = str(tmp_path / "tune.json")
     res = autotune.tune(tracer, [INCL], [FREQ], space=dict(blocks_per_cu=(0, 4),
-                                                           refill=(1,), tile=(1, 8)),
+                                                           refill=(1,), tile=(1, 8),
+                                                           order=(rt.ORDER_CENTRE,)),
                         reps=1, path=path)
     assert len(res["table"]) == 4 and res["ms"] > 0
     assert tracer.get_launch_config()["tile"] == res["config"]["tile"]
