@@ -24,7 +24,7 @@
 
 using namespace rt;
 
-static constexpr int kDefaultRefill = 1;
+static constexpr int kDefaultRefill = 4;
 static constexpr int kDefaultTile = 8;   // one wave's 64 lanes take one 8 x 8 tile
 
 namespace {
@@ -106,9 +106,11 @@ __device__ inline bool claim(const Params &P, const TraceArgs &A, int &ray)
 // at least `refill` lanes of its wave are idle; then all idle lanes claim new
 // ray ids together.  Lanes claiming together get consecutive ids (the atomic
 // is combined across the wave), i.e. neighbouring pixels, whose rays stay in
-// step for most of their length; refill = 1 refills each lane at once (best
-// balance), refill = 64 refills whole waves (most coherence).  The loop ends
-// when the queue is exhausted and every lane is idle, so every wave drains.
+// step for most of their length; refill = 1 refills each lane at once,
+// refill = 64 refills whole waves.  Waiting for a few idle lanes (default 4)
+// runs the ray set-up once for several lanes at the cost of a few idle lane
+// steps.  The loop ends when the queue is exhausted and every lane is idle,
+// so every wave drains.
 template <int NF, bool GRAD, bool DIM3>
 __global__ __launch_bounds__(64) void trace_kernel(const Params P, const TraceArgs A)
 {
