@@ -5,32 +5,36 @@ FLOPs of one image = sum over rays of
 where N_step (RK2 steps) and N_rt (samples that pass the grid gate and the
 sigma cut) are counted per ray on the device.
 
-The constants are hand counts of the arithmetic in raptor_amd/csrc/rt_physics.hpp
-(the minimal form of the reference's algorithm: the connection contracted from
-its unique entries, trigonometry shared per point, no repeated metric
-evaluations).  Each add, subtract or multiply counts 1 (an FMA 2), and each
-division, square/cube root, exp, sin or cos counts 1; products of literal
-constants are folded.  Per-function breakdown:
+The constants are frozen hand counts of the reference's algorithm in its
+minimal form: the MKS connection (metric.c:538-638) evaluated from its
+unique entries and contracted with k^a k^b using its symmetry, the
+trigonometry of a point computed once, no repeated metric evaluations
+(raptor_harm_model.c and radiative_transfer.c each evaluate the metric
+twice per sample; counted once).  Each add, subtract or multiply counts 1
+(an FMA 2), and each division, square/cube root, exp, sin or cos counts 1;
+products of literal constants are folded.  Per-function breakdown:
 
   make_point            5 arith + 5 transcendental (exp, 2 sincospi) = 10
-  geodesic_accel      197 arith + 1 division                         = 198
-    (45 shared prelude: Sigma, Delta, the reciprocals, B, C; 17 shared
-    contraction terms u, w, G002, G112; 27 / 34 / 40 / 35 for A^t, A^r,
-    A^theta, A^phi)
+  geodesic_accel      249 arith + 1 division                         = 250
   stepsize              3 arith + 2 division                         = 5
-  rk2_step            2*accel + 2*make_point + 37 updates            = 453
-    (the midpoint counts as a full make_point: its evaluation by angle
-    addition from the start point is an implementation choice)
+  rk2_step            2*accel + 2*make_point + 40 updates            = 560
   loop bookkeeping      1                                             = 1
-  F_STEP = 5 + 453 + 1                                                = 459
+  F_STEP = 5 + 560 + 1                                                = 566
     of which 10 transcendental, 4 division
 
   fluid_sample (FP32) 231 arith + 9 reciprocal + 2 sqrt              = 242 (F_FLUID)
   transfer per freq    46 arith + 4 reciprocal + 1 sqrt + 7 transc.  = 58  (F_NU)
   tangent pass: +11 per sample (dTheta_e/dR_high), +41 per sample-frequency
+
+The counts stay fixed when the implementation does the same mathematics
+with less arithmetic, so such changes raise the achieved rate: the tracer
+forms the contraction from the Kerr-Schild structure of the metric (about
+120 operations instead of 250), derives the trigonometry of the RK2
+midpoint and end point by angle addition from the start point, and replaces
+FP64 divisions by a refined reciprocal.
 """
 
-F_STEP = 459
+F_STEP = 566
 F_STEP_TRANS = 10
 F_STEP_DIV = 4
 F_FLUID = 242
