@@ -89,8 +89,10 @@ def test_host_build_multi_frequency(dump128):
 
 def test_midpoint_trigonometry_matches_full_evaluation():
     """shifted_point (Taylor + angle addition from the start point) against a
-    full exp/sincospi evaluation at the midpoint, over the step sizes the
-    integrator produces and beyond its guarded range (full fallback)."""
+    full exp/sincospi evaluation at the shifted point, over the half and full
+    steps the integrator produces (|dl k^1| <= 0.03, |dl k^2| <= 0.03
+    min(X2, 1-X2)), at their bounds, and beyond the guarded range (full
+    fallback)."""
     L = _lib()
     L.hc_shifted_point.argtypes = [ctypes.c_double] * 5 + [ctypes.c_void_p]
     rng = np.random.default_rng(11)
@@ -98,9 +100,10 @@ def test_midpoint_trigonometry_matches_full_evaluation():
     for _ in range(2000):
         X1 = rng.uniform(0.3, 9.3)
         X2 = rng.uniform(1e-4, 1 - 1e-4)
-        e1 = rng.uniform(-0.02, 0.02) * rng.choice([1, 3])
-        e2 = rng.uniform(-0.01, 0.01) * min(X2, 1 - X2) * rng.choice([1, 3])
-        h = rng.choice([0.3, 0.7, 1.0])
+        e1 = rng.choice([rng.uniform(-0.03, 0.03), 0.03, -0.03]) * rng.choice([1, 1, 3])
+        e2 = (rng.choice([rng.uniform(-0.03, 0.03), 0.03, -0.03]) * min(X2, 1 - X2) *
+              rng.choice([1, 1, 3]))
+        h = rng.choice([0.0, 0.3, 0.7, 1.0])
         L.hc_shifted_point(X1, X2, e1, e2, h, ctypes.c_void_p(out.ctypes.data))
         a, b = out[:5], out[5:]
         np.testing.assert_allclose(a, b, rtol=1e-14, atol=1e-15)
