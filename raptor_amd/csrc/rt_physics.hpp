@@ -141,11 +141,14 @@ __host__ __device__ inline Point make_point(double X1, double X2, double h)
 // e1 = dl k^1 / 2 (or dl k^1) in X1 and e2 = dl k^2 / 2 (or dl k^2) in X2,
 // with |dl k^1| <= STEPSIZE and |dl k^2| <= STEPSIZE min(X2, 1-X2) by the
 // step-size rule, so their trigonometry follows from the start point's by
-// short Taylor series and angle addition.  The truncation errors are below
-// 1e-17 within the guarded ranges; outside them (never with the reference's
-// STEPSIZE = 0.03) the point is evaluated in full.  Carried from step to step,
-// the rounding of the additions accumulates as a random walk: a ray of
-// 2000 steps keeps its trigonometry within 1e-14 of a fresh evaluation.
+// short Taylor series and angle addition.  With the reference's STEPSIZE =
+// 0.03: |e1| <= 0.03, |2 pi e2| <= 0.0943, and the theta increment
+// pi (e2 + (1-h)/(2 pi) (sin 2 pi X2' - sin 2 pi X2)) is at most
+// pi |e2| + |2 pi e2| / 2 <= 0.0943.  Within the guards (0.031, 0.095) the
+// truncation errors are below 2.2e-17; beyond them (only with a larger
+// STEPSIZE) the point is evaluated in full.  Carried from step to step, the
+// rounding of the additions accumulates as a random walk: a ray of 2000
+// steps keeps its trigonometry within 1e-14 of a fresh evaluation.
 __host__ __device__ inline double exp_small(double x)   // |x| <= 0.031: x^8/8! < 2.2e-17
 {
     double p = 1. / 5040.;
