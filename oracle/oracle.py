@@ -66,7 +66,7 @@ def lib():
         L.or_ray_path.argtypes = [P, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int]
         L.or_ray_path.restype = ctypes.c_int
         L.or_render.argtypes = [P, _dp, ctypes.c_int, _dp, _ip, ctypes.c_int, _dp, _dp, _ip,
-                                ctypes.c_int]
+                                _dp, ctypes.c_int]
         L.or_render.restype = ctypes.c_int
         L.or_transfer_step.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_double, _dp, _dp]
@@ -143,9 +143,12 @@ def ray_path(P, alpha, beta, maxpts=200000):
     return out[:n]
 
 
-def render(P, prims, freqs, pixels=None, grad=False, counters=False, nthreads=0):
+def render(P, prims, freqs, pixels=None, grad=False, counters=False, gate=False, nthreads=0):
     """Trace pixels (all by default).  Returns image [npix, nf] (I_nu, cgs),
-    optionally dimage [npix, nf, 2] and counters [npix, 4]."""
+    optionally dimage [npix, nf, 2], counters [npix, 4] (steps, samples in the
+    grid, samples past the sigma cut, escaped) and gate [npix, nf]: the
+    distance of tau from ln 1000 at the sample where it crosses that gate
+    (inf if it never does)."""
     p = np.ascontiguousarray(prims, float)
     freqs = np.ascontiguousarray(np.atleast_1d(freqs), float)
     nf = freqs.size
@@ -159,9 +162,11 @@ def render(P, prims, freqs, pixels=None, grad=False, counters=False, nthreads=0)
     img = np.zeros((npix, nf))
     dimg = np.zeros((npix, nf, 2)) if grad else None
     cnt = np.zeros((npix, 4), dtype=np.int32) if counters else None
+    gt = np.zeros((npix, nf)) if gate else None
     rc = lib().or_render(ctypes.byref(P), _d(p), nf, _d(freqs), pix_ptr, npix, _d(img),
                          _d(dimg) if grad else None,
-                         cnt.ctypes.data_as(_ip) if counters else None, nthreads)
+                         cnt.ctypes.data_as(_ip) if counters else None,
+                         _d(gt) if gate else None, nthreads)
     if rc != 0:
         raise RuntimeError("or_render failed (%d)" % rc)
     out = [img]
@@ -169,4 +174,6 @@ def render(P, prims, freqs, pixels=None, grad=False, counters=False, nthreads=0)
         out.append(dimg)
     if counters:
         out.append(cnt)
+    if gate:
+        out.append(gt)
     return out[0] if len(out) == 1 else tuple(out)

@@ -74,9 +74,14 @@ typedef struct {
 } or_params;
 
 /* per-ray counters: RK2 steps, samples inside the grid gate, samples passing
- * the sigma cut (transfer actually evaluated) */
+ * the sigma cut (transfer actually evaluated); and per frequency the distance
+ * of tau from ln 1000 at the sample where it crosses that gate
+ * (radiative_transfer.c:90-95): min(ln 1000 - tau before, tau after - ln 1000),
+ * or +inf if tau never crosses.  A transfer whose tau differs from this one
+ * by more than that margin adds or drops one sample's e^-6.9-weighted term. */
 typedef struct {
     int steps, n_grid, n_rt, escaped;
+    double gate[64];
 } or_counters;
 
 /* ---------------------------------------------------------------- units */
@@ -571,6 +576,9 @@ void or_trace_ray(const or_params *P, const double *prims, int x, int yy, int nf
     double thetadot_prev = 0.;
     int n_grid = 0, n_rt = 0, escaped = 0;
     const double lnth = log(1000.);
+    double gate[64];
+    for (int f = 0; f < nf; f++)
+        gate[f] = INFINITY;
     const double mc2_h = OR_ELECTRON_MASS * OR_C * OR_C / OR_PLANCK;
 
     while (r_current < OR_CUTOFF_OUTER && r_current > D.cutoff_inner &&
@@ -632,7 +640,10 @@ void or_trace_ray(const or_params *P, const double *prims, int x, int yy, int nf
                     double an = or_a_th(j, nu_p, F.thetae);
                     double dlf = dlabs * D.Rg / freqs[f];
                     double dtau, dIk;
+                    const double tau0 = tau[f];
                     or_transfer_step(&tau[f], &I[f], j, an, nu_p, dlf, &dtau, &dIk);
+                    if (tau0 < lnth && tau[f] >= lnth)
+                        gate[f] = fmin(lnth - tau0, tau[f] - lnth);
                     /* tangents: d ln j and d ln alpha per parameter */
                     double dlnj[2] = {0., 0.}, dlna[2] = {0., 0.};
                     if (dI && j != 0.) {
@@ -667,16 +678,19 @@ void or_trace_ray(const or_params *P, const double *prims, int x, int yy, int nf
         cnt->n_grid = n_grid;
         cnt->n_rt = n_rt;
         cnt->escaped = escaped;
+        for (int f = 0; f < nf; f++)
+            cnt->gate[f] = gate[f];
     }
 }
 
 /* core.c:74-152 calculate_image over a pixel list.
  * image[p*nf + f] = I_nu (after the nu^3 scaling of core.c:136) for pixel
- * pix[p] (index y*W + x); dimage (optional) [p][f][2]; counters optional [p][4].
+ * pix[p] (index y*W + x); dimage (optional) [p][f][2]; counters optional [p][4];
+ * gate (optional) [p][f]: the tau-gate margin of or_counters.
  * nthreads <= 0 uses the OpenMP default. */
 int or_render(const or_params *P, const double *prims, int nf, const double *freqs,
               const int *pix, int npix, double *image, double *dimage, int *counters,
-              int nthreads)
+              double *gate, int nthreads)
 {
     if (nf < 1 || nf > 64 || npix < 0)
         return -1;
@@ -707,6 +721,9 @@ int or_render(const or_params *P, const double *prims, int nf, const double *fre
             counters[p * 4 + 2] = c.n_rt;
             counters[p * 4 + 3] = c.escaped;
         }
+        if (gate)
+            for (int f = 0; f < nf; f++)
+                gate[(size_t)p * nf + f] = c.gate[f];
     }
     return 0;
 }

@@ -15,6 +15,9 @@ pytestmark = pytest.mark.gpu
 
 FLUX_TOL = 1e-4
 PIX_TOL = 1e-3
+# distance of the FP64 tau from ln 1000 at the gate crossing below which the
+# FP32 tau may decide the gate differently (its error is ~1e-6)
+GATE_EPS = 1e-4
 
 
 @pytest.fixture(scope="module")
@@ -150,19 +153,28 @@ def test_3d_dump_matches_oracle(tracer):
     assert r.max() < PIX_TOL
 
 
-def test_c1_full_size_sample(tracer, dump256):
-    """1024^2 image of the 256^2 dump: a pixel sample against the oracle and
-    size-independent properties (finite, non-negative, deterministic flux)."""
+def test_c1_full_image(tracer, dump256):
+    """The 1024^2 image of the 256^2 dump against the oracle on every pixel.
+    Pixels whose FP64 tau passes within GATE_EPS of the ln 1000 gate
+    (radiative_transfer.c:90-95) at a sample are held apart: there an FP32
+    tau (error ~1e-6) may fall on the other side of the gate and add or drop
+    one sample's exp(-6.9)-weighted term.  They must be rare; every other
+    pixel meets the per-pixel bar."""
     W = 1024
     P = _setup(tracer, dump256, W)
     img, spec = tracer.render([INCL], [FREQ])
     gpu = img[0, :, 0]
     assert np.all(np.isfinite(gpu)) and gpu.min() >= 0
-    rng = np.random.default_rng(7)
-    pix = np.sort(rng.choice(W * W, 1024, replace=False)).astype(np.int32)
-    ora = O.render(P, dump256.prims, [FREQ], pixels=pix)[:, 0]
-    r = np.abs(gpu[pix] - ora) / np.maximum(np.abs(ora), 1e-6 * gpu.max())
-    assert r.max() < PIX_TOL
+    ora, gate = O.render(P, dump256.prims, [FREQ], gate=True)
+    ora, gate = ora[:, 0], gate[:, 0]
+    r = _report("c1", gpu, ora)
+    near = gate < GATE_EPS
+    print("c1: %d pixels cross the gate, %d within %g of it (max rel %.3g there); "
+          "elsewhere max rel %.3g" % (np.isfinite(gate).sum(), near.sum(), GATE_EPS,
+                                      r[near].max() if near.any() else 0., r[~near].max()))
+    assert abs(gpu.sum() / ora.sum() - 1) < FLUX_TOL
+    assert r[~near].max() < PIX_TOL
+    assert near.sum() < 1e-4 * W * W
     img2, spec2 = tracer.render([INCL], [FREQ])
     assert spec2[0, 0] == spec[0, 0]
 
@@ -235,7 +247,8 @@ def test_probe_emission(tracer):
     for i in range(n):
         j = lib.or_j_thsynch(q[i, 0], q[i, 1], q[i, 2], q[i, 3], q[i, 4])
         a = lib.or_a_th(j, q[i, 3], q[i, 2])
-        # FP32 exp(-X^(1/3)) carries a relative error ~ X^(1/3) * 2^-24
+        # exp(-X^(1/3)) multiplies the FP32 relative error of X^(1/3) (~1e-6:
+        # the inputs' rounding and the exp2/log2 estimate) by X^(1/3)
         nuc = 4.8032068e-10 * q[i, 0] / (2 * np.pi * 9.1093826e-28 * 2.99792458e10)
         X = q[i, 3] / ((2.0 / 9.0) * nuc * q[i, 2] ** 2 * np.sqrt(1 - q[i, 1] ** 2))
         tol = 2e-5 + 1e-6 * np.cbrt(X)
