@@ -489,3 +489,30 @@ def test_harm_ascii_and_binary_round_trip(tmp_path):
     harm.write_prims_bin(str(fb), d)
     rb = harm.read_prims_bin(str(fb))
     assert np.array_equal(rb.prims, d.prims) and rb.grid == d.grid
+
+
+def test_tau_gate_margin(dump128):
+    """The oracle's tau-gate margin: no ray crosses ln 1000 in an optically
+    thin setting (small M_unit), many do in a thick one, margins are
+    non-negative, and a ray's margin is the distance from ln 1000 of tau on
+    one side of the crossing, which shrinks as the samples get denser."""
+    g = dump128.grid
+
+    def margins(M_unit, stepsize=0.03):
+        P = O.params(g, width=24, height=24, inclination=60.0, M_unit=M_unit, R_high=10.0,
+                     stepsize=stepsize)
+        img, gate = O.render(P, dump128.prims, [2.3e11], gate=True)
+        return img[:, 0], gate[:, 0]
+
+    _, thin = margins(1e17)
+    assert np.all(np.isinf(thin))
+    img, thick = margins(1e25)
+    fin = np.isfinite(thick)
+    assert fin.sum() > 50
+    assert np.all(thick[fin] >= 0)
+    # the crossing falls between two samples, so the margin is at most half
+    # of the largest optical depth a single step adds; with the step halved the
+    # median margin shrinks accordingly
+    _, fine = margins(1e25, stepsize=0.015)
+    both = fin & np.isfinite(fine)
+    assert np.median(fine[both]) < 0.75 * np.median(thick[both])
