@@ -111,8 +111,24 @@ __device__ inline bool claim(const Params &P, const TraceArgs &A, int &ray)
 // runs the ray set-up once for several lanes at the cost of a few idle lane
 // steps.  The loop ends when the queue is exhausted and every lane is idle,
 // so every wave drains.
+// One frequency (except the 3D gradient pass) fits in 168 VGPRs without
+// spilling, so those kernels are also built for 3 waves per SIMD, which hide
+// more of the FP64 dependency latency: 6% on the 1024^2 image.  A launch
+// with few rays per lane is instead bound by its longest rays, which a third
+// wave sharing the SIMD slows down (512^2: 10% slower), so the 3-wave build
+// is used from kThreeWaveRays rays per launch.  The other kernels are left to
+// the compiler, which fits them in 2 waves where it can: a forced bound only
+// trades its instruction scheduling for registers they do not need
+// (measured slower).
+static constexpr long long kThreeWaveRays = 1LL << 19;
 template <int NF, bool GRAD, bool DIM3>
-__global__ __launch_bounds__(64) void trace_kernel(const Params P, const TraceArgs A)
+constexpr bool three_waves()
+{
+    return NF == 1 && !(GRAD && DIM3);
+}
+
+template <int NF, bool GRAD, bool DIM3>
+__device__ __forceinline__ void trace_rays(const Params &P, const TraceArgs &A)
 {
     Ray<NF, GRAD> R;
     int ray = 0;
@@ -147,6 +163,18 @@ __global__ __launch_bounds__(64) void trace_kernel(const Params P, const TraceAr
             }
         }
     }
+}
+
+template <int NF, bool GRAD, bool DIM3>
+__global__ __launch_bounds__(64) void trace_kernel(const Params P, const TraceArgs A)
+{
+    trace_rays<NF, GRAD, DIM3>(P, A);
+}
+
+template <int NF, bool GRAD, bool DIM3>
+__global__ __launch_bounds__(64, 3) void trace_kernel_3w(const Params P, const TraceArgs A)
+{
+    trace_rays<NF, GRAD, DIM3>(P, A);
 }
 
 __global__ void reset_queue_kernel(unsigned int *q) { *q = 0u; }
@@ -358,7 +386,11 @@ static int make_params(rt_ctx *c, const double *incl, int ncam, const double *fr
 template <int NF, bool GRAD, bool DIM3>
 static hipError_t launch_trace(rt_ctx *c, const Params &P, const TraceArgs &A)
 {
-    auto kern = trace_kernel<NF, GRAD, DIM3>;
+    void (*kern)(const Params, const TraceArgs) = trace_kernel<NF, GRAD, DIM3>;
+    if constexpr (three_waves<NF, GRAD, DIM3>()) {
+        if (A.total >= kThreeWaveRays)
+            kern = trace_kernel_3w<NF, GRAD, DIM3>;
+    }
     int block = 64;
     int per_cu = c->cfg.blocks_per_cu;
     if (per_cu <= 0) {

@@ -83,6 +83,16 @@ def test_centre_order_covers_every_pixel_once(tracer):
             tracer.free(p)
 
 
+def test_three_wave_build_equals_two_wave_build(tracer):
+    """Launches of at least 2^19 rays run the kernel built for 3 waves per
+    SIMD, smaller ones the 2-wave build; the same rays give the same bits."""
+    _params(tracer, 512)
+    big, _ = tracer.render([INCL, 30.0], [FREQ])      # 524288 rays: 3-wave build
+    small, _ = tracer.render([INCL], [FREQ])          # 262144 rays: 2-wave build
+    assert np.array_equal(big[0], small[0])
+    assert big[0].max() > 0
+
+
 def test_c2_batched_equals_per_image(tracer):
     """Six inclinations x eight log-spaced frequencies (86-690 GHz) in one
     launch equal 48 single-image launches."""
