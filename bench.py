@@ -164,7 +164,7 @@ def main():
             "traffic": traffic,
             "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE, "
                             "copy-calibrated (profiles/trace_kernel_traffic.json)",
-            "kernel": "trace_kernel<1,false,false>",
+            "kernel": "trace_kernel_3w<1,false,false>",
             "kernel_ms": kern_ms,
             "alg_flops_per_launch": alg_flops,
             "alg_flops_fp64": f64,

@@ -45,3 +45,32 @@ def rel_err(a, b, floor_frac=1e-6):
     b = np.asarray(b, float)
     floor = floor_frac * np.max(np.abs(b))
     return np.abs(a - b) / np.maximum(np.abs(b), floor)
+
+
+# Spacetimes and step sizes beyond the headline workload: Schwarzschild, a
+# near-extremal spin, the unsqueezed grid (h = 1), a negative spin (on the
+# a = 0.5 torus's primitives: parity between two codes needs no consistent
+# fluid), and step sizes whose RK2 points leave the angle-addition range.
+SPACETIME_CASES = {
+    "a0": dict(a=0.0),
+    "a0.99": dict(a=0.99),
+    "a0.5-h1": dict(a=0.5, hslope=1.0),
+    "a-0.5": dict(a=0.5, grid_a=-0.5),
+    "step0.1": dict(stepsize=0.1),
+    "step0.01": dict(stepsize=0.01),
+}
+
+
+def spacetime_case(name, n=48):
+    """(dump, stepsize) for SPACETIME_CASES[name] on an n x n grid."""
+    import dataclasses
+    from raptor_amd import torus
+    c = dict(SPACETIME_CASES[name])
+    stepsize = c.pop("stepsize", 0.03)
+    grid_a = c.pop("grid_a", None)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        dump = torus.fishbone_moncrief(n, n, **c)
+    if grid_a is not None:
+        dump = dataclasses.replace(dump, grid=dataclasses.replace(dump.grid, a=grid_a))
+    return dump, stepsize

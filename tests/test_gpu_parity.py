@@ -7,7 +7,7 @@ near-FP64 rounding.
 import numpy as np
 import pytest
 
-from conftest import FREQ, INCL, M_UNIT, R_HIGH, rel_err
+from conftest import FREQ, INCL, M_UNIT, R_HIGH, SPACETIME_CASES, rel_err, spacetime_case
 from oracle import oracle as O
 from raptor_amd import rt
 
@@ -177,6 +177,36 @@ def test_c1_full_image(tracer, dump256):
     assert near.sum() < 1e-4 * W * W
     img2, spec2 = tracer.render([INCL], [FREQ])
     assert spec2[0, 0] == spec[0, 0]
+
+
+@pytest.mark.parametrize("case", sorted(SPACETIME_CASES))
+def test_other_spacetimes_and_step_sizes(tracer, case):
+    """Schwarzschild, a = 0.99, h = 1, negative spin, and step sizes 0.1
+    (RK2 points beyond the angle-addition range: full evaluation) and 0.01,
+    at 86 GHz with R_high = 1 (the a = 0 torus is too cold to shine at 230 GHz
+    within FP32 range): image, flux and step counts against the oracle."""
+    dump, stepsize = spacetime_case(case, 48)
+    W, nu, incl = 48, 8.6e10, (20.0 if case == "a0.99" else INCL)
+    tracer.set_params(width=W, height=W, M_unit=M_UNIT, R_high=1.0, stepsize=stepsize)
+    tracer.load_dump(dump)
+    P = O.params(dump.grid, width=W, height=W, inclination=incl, M_unit=M_UNIT, R_high=1.0,
+                 stepsize=stepsize)
+    n = W * W
+    d, dc = tracer.alloc(n * 8), tracer.alloc(n * 8)
+    try:
+        tracer.render_device([incl], [nu], d, None, dc)
+        tracer.synchronize()
+        gpu = tracer.d2h(d, (n,), np.float64)
+        cnt = tracer.d2h(dc, (n, 2), np.int32)
+    finally:
+        tracer.free(d)
+        tracer.free(dc)
+    ora, ocnt, gate = O.render(P, dump.prims, [nu], counters=True, gate=True)
+    assert ora.max() > 1e-12
+    r = _report(case, gpu, ora[:, 0])
+    assert abs(gpu.sum() / ora[:, 0].sum() - 1) < FLUX_TOL
+    assert r[~(gate[:, 0] < GATE_EPS)].max() < PIX_TOL
+    assert (cnt[:, 0] == ocnt[:, 0]).mean() > 0.99 and (cnt[:, 1] == ocnt[:, 2]).mean() > 0.99
 
 
 # ---- building blocks ------------------------------------------------------

@@ -10,7 +10,7 @@ import warnings
 import numpy as np
 import pytest
 
-from conftest import FREQ, INCL, M_UNIT, R_HIGH, rel_err
+from conftest import FREQ, INCL, M_UNIT, R_HIGH, SPACETIME_CASES, rel_err, spacetime_case
 from oracle import oracle as O
 from raptor_amd import rt
 
@@ -31,14 +31,14 @@ def _lib():
     return ctypes.CDLL(LIB)
 
 
-def _render(dump, W, incl, freqs, grad=False, R_high=R_HIGH):
+def _render(dump, W, incl, freqs, grad=False, R_high=R_HIGH, stepsize=0.03):
     L = _lib()
     g = dump.grid
     grid = rt.RtGrid(n1=g.n1, n2=g.n2, n3=g.n3, startx1=g.startx1, startx2=g.startx2,
                      startx3=g.startx3, dx1=g.dx1, dx2=g.dx2, dx3=g.dx3, a=g.a, hslope=g.hslope,
                      R0=g.R0)
     p = rt.RtParams(MBH=8.9505e39, M_unit=M_UNIT, R_high=R_high, R_low=1.0, width=W, height=W,
-                    cam_size_x=40.0, cam_size_y=40.0, stepsize=0.03, max_steps=100000)
+                    cam_size_x=40.0, cam_size_y=40.0, stepsize=stepsize, max_steps=100000)
     prims = np.ascontiguousarray(dump.prims, np.float32)
     incl = np.ascontiguousarray(incl, float)
     fr = np.ascontiguousarray(freqs, float)
@@ -107,3 +107,21 @@ def test_midpoint_trigonometry_matches_full_evaluation():
         L.hc_shifted_point(X1, X2, e1, e2, h, ctypes.c_void_p(out.ctypes.data))
         a, b = out[:5], out[5:]
         np.testing.assert_allclose(a, b, rtol=1e-14, atol=1e-15)
+
+
+@pytest.mark.parametrize("case", sorted(SPACETIME_CASES))
+def test_host_build_other_spacetimes(case):
+    """The tracer's physics on the CPU against the oracle for other spins,
+    h = 1 and step sizes beyond the angle-addition range (86 GHz, R_high = 1:
+    the a = 0 torus is too cold to shine at 230 GHz within FP32 range)."""
+    dump, stepsize = spacetime_case(case, 32)
+    W = 20
+    img, _, cnt = _render(dump, W, [INCL], [8.6e10], R_high=1.0, stepsize=stepsize)
+    P = O.params(dump.grid, width=W, height=W, inclination=INCL, M_unit=M_UNIT,
+                 R_high=1.0, stepsize=stepsize)
+    oimg, ocnt, gate = O.render(P, dump.prims, [8.6e10], counters=True, gate=True)
+    assert oimg.max() > 1e-12
+    near = gate[:, 0] < 1e-4
+    assert rel_err(img[:, 0], oimg[:, 0])[~near].max() < 1e-3
+    assert abs(img[:, 0].sum() / oimg[:, 0].sum() - 1) < 1e-4
+    assert (cnt[:, 0] == ocnt[:, 0]).mean() > 0.99
