@@ -321,11 +321,12 @@ __host__ __device__ inline void rk2_step(double y[8], Point &pt, double dl, doub
 {
     double A[4], ym[8];
     geodesic_accel(pt, a, h, y + 4, A);
+    const double hdl = 0.5 * dl;   // exact; one FMA per component
     ym[0] = y[0];
     for (int i = 1; i < 4; i++)
-        ym[i] = y[i] + dl * y[4 + i] * 0.5;
+        ym[i] = y[i] + hdl * y[4 + i];
     for (int i = 0; i < 4; i++)
-        ym[4 + i] = y[4 + i] + dl * A[i] * 0.5;
+        ym[4 + i] = y[4 + i] + hdl * A[i];
     Point pm = shifted_point(pt, ym[1], ym[2], ym[1] - y[1], ym[2] - y[2], h);
     geodesic_accel(pm, a, h, ym + 4, A);
     const double X1o = y[1], X2o = y[2];
