@@ -67,7 +67,14 @@ typedef struct {
     double cam_size_y;
     double stepsize;        /* adaptive step scale             (STEPSIZE)     */
     int max_steps;          /* step cap, 1e5 in parameters.h:182              */
+    int integrator;         /* int_method (parameters.h:186-189): RT_INT_RK2
+                               (0, the reference's default), RT_INT_RK4,
+                               RT_INT_VERLET (integrator.c:13-105)            */
 } rt_params;
+
+#define RT_INT_RK2 0
+#define RT_INT_RK4 1
+#define RT_INT_VERLET 2
 
 /* Per-ray counters written when requested: RK2 steps taken and samples for
  * which the transfer was evaluated (inside the grid and past the sigma cut). */

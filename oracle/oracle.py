@@ -27,7 +27,12 @@ class OrParams(ctypes.Structure):
         ("R_high", ctypes.c_double), ("R_low", ctypes.c_double),
         ("cam_size_x", ctypes.c_double), ("cam_size_y", ctypes.c_double),
         ("inclination", ctypes.c_double), ("stepsize", ctypes.c_double),
+        ("integrator", ctypes.c_int),
     ]
+
+
+# int_method of parameters.h:186-189
+RK2, RK4, VERLET = 0, 1, 2
 
 
 def build():
@@ -47,6 +52,7 @@ def lib():
         L.or_metric_uu.argtypes = [P, _dp, _dp]
         L.or_connection.argtypes = [P, _dp, _dp]
         L.or_rk2_step.argtypes = [P, _dp, ctypes.c_double]
+        L.or_step.argtypes = [P, _dp, ctypes.c_double]
         L.or_stepsize.argtypes = [P, _dp, _dp]
         L.or_stepsize.restype = ctypes.c_double
         L.or_theta_to_x2.argtypes = [ctypes.c_double, ctypes.c_double]
@@ -79,13 +85,13 @@ def _d(a):
 
 
 def params(grid, *, width, height, inclination, M_unit, R_high, R_low=1.0, MBH=8.9505e39,
-           cam_size=40.0, stepsize=0.03, max_steps=100000):
+           cam_size=40.0, stepsize=0.03, max_steps=100000, integrator=RK2):
     return OrParams(n1=grid.n1, n2=grid.n2, n3=grid.n3, width=width, height=height,
                     max_steps=max_steps, startx1=grid.startx1, startx2=grid.startx2,
                     startx3=grid.startx3, dx1=grid.dx1, dx2=grid.dx2, dx3=grid.dx3, a=grid.a,
                     hslope=grid.hslope, R0=grid.R0, MBH=MBH, M_unit=M_unit, R_high=R_high,
                     R_low=R_low, cam_size_x=cam_size, cam_size_y=cam_size,
-                    inclination=inclination, stepsize=stepsize)
+                    inclination=inclination, stepsize=stepsize, integrator=integrator)
 
 
 def metric_dd(P, X):
@@ -109,6 +115,13 @@ def connection(P, X):
 def rk2_step(P, y, dl):
     y = np.array(y, float)
     lib().or_rk2_step(ctypes.byref(P), _d(y), dl)
+    return y
+
+
+def step(P, y, dl):
+    """One step of the integrator P.integrator selects (RK2, RK4, VERLET)."""
+    y = np.array(y, float)
+    lib().or_step(ctypes.byref(P), _d(y), dl)
     return y
 
 

@@ -71,6 +71,7 @@ typedef struct {
     double cam_size_x, cam_size_y;
     double inclination;       /* degrees */
     double stepsize;          /* STEPSIZE, model.in:12 */
+    int integrator;           /* int_method (parameters.h:186-189): 0 RK2, 1 RK4, 2 Verlet */
 } or_params;
 
 /* per-ray counters: RK2 steps, samples inside the grid gate, samples passing
@@ -265,6 +266,55 @@ void or_rk2_step(const or_params *P, double y[8], double dt)
         dx1[i] = dt * f[i];
     for (int i = 0; i < 8; i++)
         y[i] = y[i] + dx1[i];
+}
+
+/* integrator.c:13-39 rk4_step: classical RK4 with weights (1/2, 1/2, 1) and
+ * the update (dx0 + 2 dx1 + 2 dx2 + dx3) / 6 */
+void or_rk4_step(const or_params *P, double y[8], double dt)
+{
+    const double w[4] = {0.5, 0.5, 1., 0.};
+    double ys[8], f[8], dx[4][8];
+    memcpy(ys, y, sizeof ys);
+    for (int q = 0; q < 4; q++) {
+        or_f_geodesic(P, ys, f);
+        for (int i = 0; i < 8; i++) {
+            dx[q][i] = dt * f[i];
+            ys[i] = y[i] + dx[q][i] * w[q];
+        }
+    }
+    for (int i = 0; i < 8; i++)
+        y[i] = y[i] + 1. / 6. * (dx[0][i] + dx[1][i] * 2. + dx[2][i] * 2. + dx[3][i]);
+}
+
+/* integrator.c:71-105 verlet_step, velocity Verlet (Dolence et al. 2009
+ * eq. 14a-d): X += dl k + dl^2 A / 2, k~ = k + dl A, then
+ * k += dl (A(X, k) + A(X', k~)) / 2 */
+void or_verlet_step(const or_params *P, double y[8], double dl)
+{
+    double ys[8], f[8], A0[4];
+    memcpy(ys, y, sizeof ys);
+    or_f_geodesic(P, ys, f);
+    for (int i = 0; i < 4; i++) {
+        ys[i] += dl * ys[i + 4] + 0.5 * dl * dl * f[i + 4];
+        ys[i + 4] = ys[i + 4] + f[i + 4] * dl;
+        A0[i] = f[i + 4];
+    }
+    or_f_geodesic(P, ys, f);
+    for (int i = 0; i < 4; i++) {
+        y[i] = ys[i];
+        y[i + 4] += 0.5 * (A0[i] + f[i + 4]) * dl;
+    }
+}
+
+/* integrator.c:258-263: the stepper chosen by int_method */
+void or_step(const or_params *P, double y[8], double dl)
+{
+    if (P->integrator == 1)
+        or_rk4_step(P, y, dl);
+    else if (P->integrator == 2)
+        or_verlet_step(P, y, dl);
+    else
+        or_rk2_step(P, y, dl);
 }
 
 /* integrator.c:109-122 stepsize */
@@ -591,7 +641,7 @@ void or_trace_ray(const or_params *P, const double *prims, int x, int yy, int nf
             TERMINATE = 1;
         double dl = or_stepsize(P, y, y + 4);
         double dlabs = fabs(dl);
-        or_rk2_step(P, y, dl);
+        or_step(P, y, dl);
         if (y[1] > D.stopx1 && y[5] < 0) {
             escaped = 1;
             break;
@@ -779,7 +829,7 @@ int or_ray_path(const or_params *P, double alpha, double beta, double *out, int 
     int steps = 0;
     while (r_current < OR_CUTOFF_OUTER && r_current > D.cutoff_inner && steps < P->max_steps) {
         double dl = or_stepsize(P, y, y + 4);
-        or_rk2_step(P, y, dl);
+        or_step(P, y, dl);
         if (n < maxpts)
             memcpy(out + 8 * n++, y, sizeof y);
         if (y[1] > D.stopx1 && y[5] < 0)

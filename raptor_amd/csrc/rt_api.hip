@@ -121,13 +121,13 @@ __device__ inline bool claim(const Params &P, const TraceArgs &A, int &ray)
 // trades its instruction scheduling for registers they do not need
 // (measured slower).
 static constexpr long long kThreeWaveRays = 1LL << 19;
-template <int NF, bool GRAD, bool DIM3>
+template <int NF, bool GRAD, bool DIM3, int INT>
 constexpr bool three_waves()
 {
-    return NF == 1 && !(GRAD && DIM3);
+    return INT == kRK2 && NF == 1 && !(GRAD && DIM3);
 }
 
-template <int NF, bool GRAD, bool DIM3>
+template <int NF, bool GRAD, bool DIM3, int INT>
 __device__ __forceinline__ void trace_rays(const Params &P, const TraceArgs &A)
 {
     Ray<NF, GRAD> R;
@@ -140,7 +140,7 @@ __device__ __forceinline__ void trace_rays(const Params &P, const TraceArgs &A)
         if (active) {
             bool cont = ray_alive<NF, GRAD>(P, R);
             if (cont)
-                cont = ray_step<NF, GRAD, DIM3>(P, A.prims, R);
+                cont = ray_step<NF, GRAD, DIM3, INT>(P, A.prims, R);
             if (!cont) {
                 write_ray<NF, GRAD, DIM3>(P, A, ray, R);
                 active = false;
@@ -165,16 +165,16 @@ __device__ __forceinline__ void trace_rays(const Params &P, const TraceArgs &A)
     }
 }
 
-template <int NF, bool GRAD, bool DIM3>
+template <int NF, bool GRAD, bool DIM3, int INT>
 __global__ __launch_bounds__(64) void trace_kernel(const Params P, const TraceArgs A)
 {
-    trace_rays<NF, GRAD, DIM3>(P, A);
+    trace_rays<NF, GRAD, DIM3, INT>(P, A);
 }
 
-template <int NF, bool GRAD, bool DIM3>
+template <int NF, bool GRAD, bool DIM3, int INT>
 __global__ __launch_bounds__(64, 3) void trace_kernel_3w(const Params P, const TraceArgs A)
 {
-    trace_rays<NF, GRAD, DIM3>(P, A);
+    trace_rays<NF, GRAD, DIM3, INT>(P, A);
 }
 
 __global__ void reset_queue_kernel(unsigned int *q) { *q = 0u; }
@@ -383,13 +383,13 @@ static int make_params(rt_ctx *c, const double *incl, int ncam, const double *fr
     return RT_OK;
 }
 
-template <int NF, bool GRAD, bool DIM3>
+template <int NF, bool GRAD, bool DIM3, int INT>
 static hipError_t launch_trace(rt_ctx *c, const Params &P, const TraceArgs &A)
 {
-    void (*kern)(const Params, const TraceArgs) = trace_kernel<NF, GRAD, DIM3>;
-    if constexpr (three_waves<NF, GRAD, DIM3>()) {
+    void (*kern)(const Params, const TraceArgs) = trace_kernel<NF, GRAD, DIM3, INT>;
+    if constexpr (three_waves<NF, GRAD, DIM3, INT>()) {
         if (A.total >= kThreeWaveRays)
-            kern = trace_kernel_3w<NF, GRAD, DIM3>;
+            kern = trace_kernel_3w<NF, GRAD, DIM3, INT>;
     }
     int block = 64;
     int per_cu = c->cfg.blocks_per_cu;
@@ -407,16 +407,26 @@ static hipError_t launch_trace(rt_ctx *c, const Params &P, const TraceArgs &A)
     return hipGetLastError();
 }
 
-template <bool GRAD, bool DIM3>
+template <bool GRAD, bool DIM3, int INT>
 static hipError_t dispatch_nf(rt_ctx *c, const Params &P, const TraceArgs &A)
 {
     if (P.nf == 1)
-        return launch_trace<1, GRAD, DIM3>(c, P, A);
+        return launch_trace<1, GRAD, DIM3, INT>(c, P, A);
     if (P.nf <= 4)
-        return launch_trace<4, GRAD, DIM3>(c, P, A);
+        return launch_trace<4, GRAD, DIM3, INT>(c, P, A);
     if (P.nf <= 8)
-        return launch_trace<8, GRAD, DIM3>(c, P, A);
-    return launch_trace<16, GRAD, DIM3>(c, P, A);
+        return launch_trace<8, GRAD, DIM3, INT>(c, P, A);
+    return launch_trace<16, GRAD, DIM3, INT>(c, P, A);
+}
+
+template <bool GRAD, bool DIM3>
+static hipError_t dispatch_int(rt_ctx *c, const Params &P, const TraceArgs &A)
+{
+    if (c->params.integrator == RT_INT_RK4)
+        return dispatch_nf<GRAD, DIM3, kRK4>(c, P, A);
+    if (c->params.integrator == RT_INT_VERLET)
+        return dispatch_nf<GRAD, DIM3, kVerlet>(c, P, A);
+    return dispatch_nf<GRAD, DIM3, kRK2>(c, P, A);
 }
 
 static int ensure_buffer(rt_ctx *c, double **buf, size_t *cap, size_t bytes)
@@ -556,9 +566,9 @@ static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *fr
     hipError_t e;
     bool d3 = c->grid.n3 > 1;
     if (grad)
-        e = d3 ? dispatch_nf<true, true>(c, P, A) : dispatch_nf<true, false>(c, P, A);
+        e = d3 ? dispatch_int<true, true>(c, P, A) : dispatch_int<true, false>(c, P, A);
     else
-        e = d3 ? dispatch_nf<false, true>(c, P, A) : dispatch_nf<false, false>(c, P, A);
+        e = d3 ? dispatch_int<false, true>(c, P, A) : dispatch_int<false, false>(c, P, A);
     HIPCHK(c, e);
     if (!c->capturing)
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -691,6 +701,9 @@ int rt_set_params(rt_ctx *c, const rt_params *p)
     if (!(p->MBH > 0.) || !(p->M_unit > 0.) || !(p->stepsize > 0.) || p->max_steps < 1 ||
         !(p->cam_size_x > 0.) || !(p->cam_size_y > 0.))
         return fail(c, RT_E_INVALID, "MBH, M_unit, stepsize, max_steps and camera size must be positive");
+    if (p->integrator < RT_INT_RK2 || p->integrator > RT_INT_VERLET)
+        return fail(c, RT_E_INVALID, "integrator must be RT_INT_RK2, RT_INT_RK4 or RT_INT_VERLET, "
+                                     "got %d", p->integrator);
     c->params = *p;
     c->have_params = true;
     return RT_OK;

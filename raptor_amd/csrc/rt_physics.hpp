@@ -337,6 +337,79 @@ __host__ __device__ inline void rk2_step(double y[8], Point &pt, double dl, doub
     pt = shifted_point(pt, y[1], y[2], y[1] - X1o, y[2] - X2o, h);
 }
 
+// The steppers of integrator.c:258-263, chosen by int_method
+// (parameters.h:186-189); RK2 is the reference's default.
+constexpr int kRK2 = 0, kRK4 = 1, kVerlet = 2;
+
+// integrator.c:13-39: classical RK4, stages at y, y + dx0/2, y + dx1/2 and
+// y + dx2 with dx_q = dl f(stage q), update y + (dx0 + 2 dx1 + 2 dx2 + dx3)/6,
+// summed in that order.  Each stage point's trigonometry follows from the
+// step's start point by angle addition (full evaluation beyond the guards).
+__host__ __device__ inline void rk4_step(double y[8], Point &pt, double dl, double a, double h)
+{
+    const double w[3] = {0.5, 0.5, 1.};
+    double ys[8], S[8], A[4];
+    for (int i = 0; i < 8; i++)
+        ys[i] = y[i];
+    Point ps = pt;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        geodesic_accel(ps, a, h, ys + 4, A);
+        double dx[8];
+        for (int i = 1; i < 4; i++)
+            dx[i] = dl * ys[4 + i];
+        for (int i = 0; i < 4; i++)
+            dx[4 + i] = dl * A[i];
+        const double c = (q == 1 || q == 2) ? 2. : 1.;
+        for (int i = 1; i < 8; i++)
+            S[i] = q == 0 ? dx[i] : S[i] + dx[i] * c;
+        if (q < 3) {
+            for (int i = 1; i < 8; i++)
+                ys[i] = y[i] + dx[i] * w[q];
+            ps = shifted_point(pt, ys[1], ys[2], ys[1] - y[1], ys[2] - y[2], h);
+        }
+    }
+    const double X1o = y[1], X2o = y[2];
+    for (int i = 1; i < 8; i++)
+        y[i] = y[i] + 1. / 6. * S[i];
+    pt = shifted_point(pt, y[1], y[2], y[1] - X1o, y[2] - X2o, h);
+}
+
+// integrator.c:71-105: velocity Verlet (Dolence et al. 2009, eq. 14a-d):
+// X' = X + dl k + dl^2 A(X, k)/2, k~ = k + dl A(X, k),
+// k' = k + dl (A(X, k) + A(X', k~))/2.  The second acceleration is at the
+// new point, whose trigonometry the next step reuses.
+__host__ __device__ inline void verlet_step(double y[8], Point &pt, double dl, double a,
+                                            double h)
+{
+    double A0[4], A1[4], ys[8];
+    geodesic_accel(pt, a, h, y + 4, A0);
+    ys[0] = y[0];
+    for (int i = 1; i < 4; i++)
+        ys[i] = y[i] + (dl * y[4 + i] + 0.5 * dl * dl * A0[i]);
+    for (int i = 0; i < 4; i++)
+        ys[4 + i] = y[4 + i] + A0[i] * dl;
+    const Point pn = shifted_point(pt, ys[1], ys[2], ys[1] - y[1], ys[2] - y[2], h);
+    geodesic_accel(pn, a, h, ys + 4, A1);
+    for (int i = 1; i < 4; i++)
+        y[i] = ys[i];
+    for (int i = 0; i < 4; i++)
+        y[4 + i] += 0.5 * (A0[i] + A1[i]) * dl;
+    pt = pn;
+}
+
+template <int INT>
+__host__ __device__ inline void geodesic_step(double y[8], Point &pt, double dl, double a,
+                                              double h)
+{
+    if (INT == kRK4)
+        rk4_step(y, pt, dl, a, h);
+    else if (INT == kVerlet)
+        verlet_step(y, pt, dl, a, h);
+    else
+        rk2_step(y, pt, dl, a, h);
+}
+
 // Pixel part of initialize_photon (metric.c:650-727): k_mu from impact
 // parameters, raised with the BL metric, converted to KS and to MKS X2.
 __host__ __device__ inline void init_photon(const Camera &cm, double alpha, double beta,
@@ -761,7 +834,7 @@ __host__ __device__ inline bool ray_alive(const Params &P, const Ray<NF, GRAD> &
 // One iteration of integrate_geodesic's loop (integrator.c:243-284).
 // Returns false when the ray escaped through the outer grid edge (the
 // `break` at integrator.c:271-272).
-template <int NF, bool GRAD, bool DIM3>
+template <int NF, bool GRAD, bool DIM3, int INT = kRK2>
 __host__ __device__ inline bool ray_step(const Params &P, const float4 *__restrict__ prims,
                                          Ray<NF, GRAD> &R)
 {
@@ -773,7 +846,7 @@ __host__ __device__ inline bool ray_step(const Params &P, const float4 *__restri
     if ((R.beta_sign < 0 && R.turns > kMaxOrder) || (R.beta_sign > 0 && R.turns > kMaxOrder + 1))
         R.terminate = true;
     double dl = stepsize(R.y, P.stepsize);
-    rk2_step(R.y, R.pt, dl, P.a, P.hslope);
+    geodesic_step<INT>(R.y, R.pt, dl, P.a, P.hslope);
     if (R.y[1] > P.stopx1 && R.y[5] < 0.)
         return false;
     if (R.y[1] < P.stopx1 && R.y[1] > P.startx1 && R.y[2] > P.startx2 && R.y[2] < P.stopx2) {

@@ -77,7 +77,8 @@ int rio_parse_args(int argc, char **argv, rio_model *m, char *err, size_t errlen
     if (argc < 8)
         return seterr(err, errlen,
                       "usage: %s model.in dumpfile M_UNIT INCLINATION R_HIGH R_LOW TIME_INIT "
-                      "[--freqs f1,f2,..] [--nfreq N] [--incl i1,i2,..] [--device D]",
+                      "[--freqs f1,f2,..] [--nfreq N] [--incl i1,i2,..] [--device D] "
+                      "[--integrator rk2|rk4|verlet]",
                       argc ? argv[0] : "raptor");
     if (strlen(argv[2]) >= sizeof m->GRMHD_FILE)
         return seterr(err, errlen, "dump file name too long");
@@ -93,6 +94,7 @@ int rio_parse_args(int argc, char **argv, rio_model *m, char *err, size_t errlen
     m->n_cam = 1;
     m->incl[0] = m->INCLINATION;
     m->device = 0;
+    m->integrator = RT_INT_RK2;
     for (int i = 8; i < argc; i++) {
         if (i + 1 >= argc)
             return seterr(err, errlen, "flag %s needs a value", argv[i]);
@@ -116,6 +118,15 @@ int rio_parse_args(int argc, char **argv, rio_model *m, char *err, size_t errlen
             m->n_cam = n;
         } else if (!strcmp(argv[i - 1], "--device")) {
             m->device = atoi(v);
+        } else if (!strcmp(argv[i - 1], "--integrator")) {
+            if (!strcmp(v, "rk2"))
+                m->integrator = RT_INT_RK2;
+            else if (!strcmp(v, "rk4"))
+                m->integrator = RT_INT_RK4;
+            else if (!strcmp(v, "verlet"))
+                m->integrator = RT_INT_VERLET;
+            else
+                return seterr(err, errlen, "--integrator must be rk2, rk4 or verlet");
         } else {
             return seterr(err, errlen, "unknown flag %s", argv[i - 1]);
         }

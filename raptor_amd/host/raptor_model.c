@@ -29,6 +29,7 @@ static void fill_params(const rio_model *m, rt_params *p)
     p->cam_size_y = m->CAM_SIZE_Y;
     p->stepsize = m->STEPSIZE;
     p->max_steps = 100000; /* parameters.h:182 */
+    p->integrator = m->integrator;
 }
 
 int init_model(rt_ctx *ctx, const rio_model *m, const rio_dump *d)

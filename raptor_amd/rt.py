@@ -39,13 +39,17 @@ class RtParams(ctypes.Structure):
                 ("R_high", ctypes.c_double), ("R_low", ctypes.c_double),
                 ("width", ctypes.c_int), ("height", ctypes.c_int),
                 ("cam_size_x", ctypes.c_double), ("cam_size_y", ctypes.c_double),
-                ("stepsize", ctypes.c_double), ("max_steps", ctypes.c_int)]
+                ("stepsize", ctypes.c_double), ("max_steps", ctypes.c_int),
+                ("integrator", ctypes.c_int)]
 
 
 class RtLaunchConfig(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
                 ("refill", ctypes.c_int), ("tile", ctypes.c_int), ("order", ctypes.c_int)]
 
+
+# rt_params.integrator (include/raptor_rt.h): int_method of parameters.h:186-189
+INT_RK2, INT_RK4, INT_VERLET = 0, 1, 2
 
 # claim orders of rt_launch_config.order (include/raptor_rt.h)
 ORDER_CENTRE = 1
@@ -161,11 +165,12 @@ class Tracer:
 
     # -- set-up ---------------------------------------------------------------
     def set_params(self, *, width, height, M_unit, R_high, R_low=1.0, MBH=8.9505e39,
-                   cam_size=40.0, cam_size_y=None, stepsize=0.03, max_steps=100000):
+                   cam_size=40.0, cam_size_y=None, stepsize=0.03, max_steps=100000,
+                   integrator=INT_RK2):
         p = RtParams(MBH=MBH, M_unit=M_unit, R_high=R_high, R_low=R_low, width=width,
                      height=height, cam_size_x=cam_size,
                      cam_size_y=cam_size if cam_size_y is None else cam_size_y,
-                     stepsize=stepsize, max_steps=max_steps)
+                     stepsize=stepsize, max_steps=max_steps, integrator=integrator)
         self._check(self.lib.rt_set_params(self.ctx, ctypes.byref(p)))
         self.width, self.height = width, height
         self.params = p

@@ -21,11 +21,12 @@ static inline __host__ void sincospi(double x, double *s, double *c)
 #include "../../include/raptor_rt.h"
 #include "../../raptor_amd/csrc/rt_physics.hpp"
 
+#include <type_traits>
 #include <vector>
 
 using namespace rt;
 
-template <int NF, bool GRAD>
+template <int NF, bool GRAD, int INT>
 static void run(const Params &P, const float4 *prims, int total, double *img, double *dimg,
                 int *cnt)
 {
@@ -34,8 +35,8 @@ static void run(const Params &P, const float4 *prims, int total, double *img, do
         ray_init<NF, GRAD>(P, ray / P.npix, ray % P.npix, R);
         bool dim3 = P.n3 > 1;
         while (ray_alive<NF, GRAD>(P, R)) {
-            bool cont = dim3 ? ray_step<NF, GRAD, true>(P, prims, R)
-                             : ray_step<NF, GRAD, false>(P, prims, R);
+            bool cont = dim3 ? ray_step<NF, GRAD, true, INT>(P, prims, R)
+                             : ray_step<NF, GRAD, false, INT>(P, prims, R);
             if (!cont)
                 break;
         }
@@ -71,13 +72,22 @@ extern "C" int hc_render(const rt_params *p, const rt_grid *g, const float *soa,
                                      soa[7 * ncell + c]);
     }
     int total = ncam * P->npix;
-    if (nf == 1) {
-        if (dimg) run<1, true>(*P, aos.data(), total, img, dimg, cnt);
-        else run<1, false>(*P, aos.data(), total, img, dimg, cnt);
-    } else {
-        if (dimg) run<16, true>(*P, aos.data(), total, img, dimg, cnt);
-        else run<16, false>(*P, aos.data(), total, img, dimg, cnt);
-    }
+    auto go = [&](auto integ) {
+        constexpr int INT = decltype(integ)::value;
+        if (nf == 1) {
+            if (dimg) run<1, true, INT>(*P, aos.data(), total, img, dimg, cnt);
+            else run<1, false, INT>(*P, aos.data(), total, img, dimg, cnt);
+        } else {
+            if (dimg) run<16, true, INT>(*P, aos.data(), total, img, dimg, cnt);
+            else run<16, false, INT>(*P, aos.data(), total, img, dimg, cnt);
+        }
+    };
+    if (p->integrator == RT_INT_RK4)
+        go(std::integral_constant<int, kRK4>());
+    else if (p->integrator == RT_INT_VERLET)
+        go(std::integral_constant<int, kVerlet>());
+    else
+        go(std::integral_constant<int, kRK2>());
     delete P;
     return 0;
 }

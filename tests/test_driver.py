@@ -50,6 +50,9 @@ def test_usage_and_argument_errors(driver, tmp_path):
     r = _run(driver, ["model.in", "d", "1e23", "60", "10", "1", "0", "--freqs", "1e11,abc"],
              tmp_path)
     assert r.returncode == 2 and "--freqs" in r.stderr
+    r = _run(driver, ["model.in", "d", "1e23", "60", "10", "1", "0", "--integrator", "euler"],
+             tmp_path)
+    assert r.returncode == 2 and "--integrator" in r.stderr
 
 
 def test_model_in_and_dump_errors(driver, tmp_path):
@@ -131,3 +134,30 @@ def test_driver_binary_3d_dump_with_lists(driver, tmp_path):
         want = _expected_files(tmp_path, img[k], W, H, freqs, inc, "dump.bin", jf)
         for name, text in want.items():
             assert (tmp_path / "output" / name).read_text() == text, name
+
+
+@pytest.mark.gpu
+def test_driver_integrator_flag(driver, tmp_path):
+    """--integrator rk4 traces with RK4 (int_method): the image equals the
+    library's RK4 render and differs from the RK2 one."""
+    from raptor_amd import rt
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        d = torus.fishbone_moncrief(48, 48)
+    harm.write_harm_ascii(str(tmp_path / "dump.txt"), d)
+    W, H = 16, 16
+    (tmp_path / "model.in").write_text(MODEL_IN.format(W=W, H=H))
+    r = _run(driver, ["model.in", "dump.txt", "%g" % M_UNIT, "60", "%g" % R_HIGH, "1", "0",
+                      "--integrator", "rk4"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    with rt.Tracer(0) as tr:
+        tr.set_params(width=W, height=H, M_unit=M_UNIT, R_high=R_HIGH, integrator=rt.INT_RK4)
+        tr.load_dump(d)
+        img4, _ = tr.render([60.0], [2.3e11])
+        tr.set_params(width=W, height=H, M_unit=M_UNIT, R_high=R_HIGH)
+        img2, _ = tr.render([60.0], [2.3e11])
+        jf = tr.jansky_factor()
+    want = _expected_files(tmp_path, img4[0], W, H, [2.3e11], 60.0, "dump.txt", jf)
+    for name, text in want.items():
+        assert (tmp_path / "output" / name).read_text() == text, name
+    assert not np.array_equal(img4, img2)

@@ -209,6 +209,37 @@ def test_other_spacetimes_and_step_sizes(tracer, case):
     assert (cnt[:, 0] == ocnt[:, 0]).mean() > 0.99 and (cnt[:, 1] == ocnt[:, 2]).mean() > 0.99
 
 
+@pytest.mark.parametrize("integrator", [rt.INT_RK4, rt.INT_VERLET])
+def test_other_integrators(tracer, dump128, integrator):
+    """RK4 and velocity Verlet (integrator.c:13-105, chosen by int_method):
+    image, flux and step counts against the oracle's."""
+    W = 64
+    tracer.set_params(width=W, height=W, M_unit=M_UNIT, R_high=R_HIGH, integrator=integrator)
+    tracer.load_dump(dump128)
+    P = O.params(dump128.grid, width=W, height=W, inclination=INCL, M_unit=M_UNIT,
+                 R_high=R_HIGH, integrator=integrator)
+    n = W * W
+    d, dc = tracer.alloc(n * 8), tracer.alloc(n * 8)
+    try:
+        tracer.render_device([INCL], [FREQ], d, None, dc)
+        tracer.synchronize()
+        gpu = tracer.d2h(d, (n,), np.float64)
+        cnt = tracer.d2h(dc, (n, 2), np.int32)
+    finally:
+        tracer.free(d)
+        tracer.free(dc)
+    ora, ocnt, gate = O.render(P, dump128.prims, [FREQ], counters=True, gate=True)
+    r = _report("integrator %d" % integrator, gpu, ora[:, 0])
+    assert abs(gpu.sum() / ora[:, 0].sum() - 1) < FLUX_TOL
+    assert r[~(gate[:, 0] < GATE_EPS)].max() < PIX_TOL
+    assert (cnt[:, 0] == ocnt[:, 0]).mean() > 0.99 and (cnt[:, 1] == ocnt[:, 2]).mean() > 0.99
+
+
+def test_bad_integrator_rejected(tracer):
+    with pytest.raises(rt.RtError, match="integrator"):
+        tracer.set_params(width=8, height=8, M_unit=M_UNIT, R_high=R_HIGH, integrator=3)
+
+
 # ---- building blocks ------------------------------------------------------
 
 

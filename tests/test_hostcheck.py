@@ -31,14 +31,15 @@ def _lib():
     return ctypes.CDLL(LIB)
 
 
-def _render(dump, W, incl, freqs, grad=False, R_high=R_HIGH, stepsize=0.03):
+def _render(dump, W, incl, freqs, grad=False, R_high=R_HIGH, stepsize=0.03, integrator=0):
     L = _lib()
     g = dump.grid
     grid = rt.RtGrid(n1=g.n1, n2=g.n2, n3=g.n3, startx1=g.startx1, startx2=g.startx2,
                      startx3=g.startx3, dx1=g.dx1, dx2=g.dx2, dx3=g.dx3, a=g.a, hslope=g.hslope,
                      R0=g.R0)
     p = rt.RtParams(MBH=8.9505e39, M_unit=M_UNIT, R_high=R_high, R_low=1.0, width=W, height=W,
-                    cam_size_x=40.0, cam_size_y=40.0, stepsize=stepsize, max_steps=100000)
+                    cam_size_x=40.0, cam_size_y=40.0, stepsize=stepsize, max_steps=100000,
+                    integrator=integrator)
     prims = np.ascontiguousarray(dump.prims, np.float32)
     incl = np.ascontiguousarray(incl, float)
     fr = np.ascontiguousarray(freqs, float)
@@ -125,3 +126,23 @@ def test_host_build_other_spacetimes(case):
     assert rel_err(img[:, 0], oimg[:, 0])[~near].max() < 1e-3
     assert abs(img[:, 0].sum() / oimg[:, 0].sum() - 1) < 1e-4
     assert (cnt[:, 0] == ocnt[:, 0]).mean() > 0.99
+
+
+@pytest.mark.parametrize("integrator", [O.RK4, O.VERLET])
+def test_host_build_other_integrators(dump128, integrator):
+    """RK4 and velocity Verlet (integrator.c:13-105), the steppers int_method
+    selects, on the CPU build against the oracle's."""
+    W = 24
+    img, _, cnt = _render(dump128, W, [INCL], [FREQ], integrator=integrator)
+    P = O.params(dump128.grid, width=W, height=W, inclination=INCL, M_unit=M_UNIT,
+                 R_high=R_HIGH, integrator=integrator)
+    oimg, ocnt, gate = O.render(P, dump128.prims, [FREQ], counters=True, gate=True)
+    near = gate[:, 0] < 1e-4
+    assert rel_err(img[:, 0], oimg[:, 0])[~near].max() < 1e-3
+    assert abs(img[:, 0].sum() / oimg[:, 0].sum() - 1) < 1e-4
+    assert (cnt[:, 0] == ocnt[:, 0]).mean() > 0.99
+    # a different stepper gives a measurably different image
+    P2 = O.params(dump128.grid, width=W, height=W, inclination=INCL, M_unit=M_UNIT,
+                  R_high=R_HIGH)
+    rk2 = O.render(P2, dump128.prims, [FREQ])
+    assert np.abs(rk2[:, 0] - oimg[:, 0]).max() > 1e-6 * np.abs(oimg).max()

@@ -168,6 +168,29 @@ def test_conserved_quantities_and_second_order_convergence():
         assert 3.5 < ratio < 4.5, (d1, d2)
 
 
+def test_rk4_fourth_order_convergence():
+    """integrator.c:13-39 RK4: the conserved-quantity drift falls ~16x per
+    halving of the step scale (a wrong weight or a dropped stage would show
+    second or third order)."""
+    d1 = _drift(_P(stepsize=0.015, integrator=O.RK4), 4.5, 3.0)
+    d2 = _drift(_P(stepsize=0.0075, integrator=O.RK4), 4.5, 3.0)
+    assert d1[0] < 1e-7
+    for a, b in zip(d1[:3], d2[:3]):
+        assert 14.0 < a / b < 19.0, (a, b)
+
+
+def test_verlet_second_order_convergence():
+    """integrator.c:71-105 velocity Verlet: second order (drift ~4x per
+    halving), and a different error than RK2's on the same ray."""
+    P1 = _P(stepsize=0.03, integrator=O.VERLET)
+    d1 = _drift(P1, 4.5, 3.0)
+    d2 = _drift(_P(stepsize=0.015, integrator=O.VERLET), 4.5, 3.0)
+    for a, b in zip(d1[:3], d2[:3]):
+        assert 3.5 < a / b < 4.5, (a, b)
+    r1 = _drift(_P(stepsize=0.03), 4.5, 3.0)
+    assert abs(d1[0] / r1[0] - 1) > 0.5
+
+
 def test_null_condition_along_ray():
     P = _P()
     path = O.ray_path(P, 2.0, -6.0)
