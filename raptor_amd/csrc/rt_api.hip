@@ -311,13 +311,18 @@ struct rt_ctx {
     float4 *d_prims = nullptr;
     unsigned int *d_queue = nullptr;
     // Centre-out claim orders, one per image shape (W, H, tile, pixel size
-    // bits).  A table is never changed or freed before rt_destroy, because
-    // captured graphs keep its address.
+    // bits); beyond kMaxOrderTables the least recently used one is freed.
+    // A table is never changed in place, and one that a captured graph
+    // launches with is pinned until that graph is destroyed.
     struct OrderTable {
         long long key[5];
         int *d;
+        int refs;                  // live graphs that use the table
+        unsigned long long used;   // last use, for eviction
     };
     std::vector<OrderTable> orders;
+    unsigned long long use_clock = 0;
+    std::vector<int *> capture_orders;   // tables used by the open capture
     double *d_image = nullptr, *d_dimage = nullptr;
     size_t image_cap = 0, dimage_cap = 0;
     double last_ms = 0.;
@@ -330,7 +335,10 @@ struct rt_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int device = 0;
+    std::vector<int *> orders;   // claim-order tables pinned for the graph's launches
 };
+
+static constexpr size_t kMaxOrderTables = 8;
 
 static char g_create_err[512];
 
@@ -455,8 +463,12 @@ static int ensure_order(rt_ctx *c, int W, int H, int T, double sx, double sy, co
     memcpy(&bx, &sx, sizeof bx);
     memcpy(&by, &sy, sizeof by);
     const long long key[5] = {W, H, T, bx, by};
-    for (const auto &t : c->orders) {
+    for (auto &t : c->orders) {
         if (memcmp(key, t.key, sizeof key) == 0) {
+            t.used = ++c->use_clock;
+            if (c->capturing && std::find(c->capture_orders.begin(), c->capture_orders.end(),
+                                          t.d) == c->capture_orders.end())
+                c->capture_orders.push_back(t.d);
             *out = t.d;
             return RT_OK;
         }
@@ -464,6 +476,20 @@ static int ensure_order(rt_ctx *c, int W, int H, int T, double sx, double sy, co
     if (c->capturing)
         return fail(c, RT_E_STATE, "the claim order for a new image shape is built by an eager "
                                    "render; render once before capturing");
+    if (c->orders.size() >= kMaxOrderTables) {
+        // free the least recently used table that no graph pins, once the
+        // launches already queued on the stream (which may read it) are done
+        size_t victim = c->orders.size();
+        for (size_t i = 0; i < c->orders.size(); i++)
+            if (c->orders[i].refs == 0 &&
+                (victim == c->orders.size() || c->orders[i].used < c->orders[victim].used))
+                victim = i;
+        if (victim < c->orders.size()) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            HIPCHK(c, hipFree(c->orders[victim].d));
+            c->orders.erase(c->orders.begin() + (long)victim);
+        }
+    }
     struct Item {
         double d;
         int tile, pix;
@@ -503,6 +529,8 @@ static int ensure_order(rt_ctx *c, int W, int H, int T, double sx, double sy, co
     rt_ctx::OrderTable t;
     memcpy(t.key, key, sizeof key);
     t.d = nullptr;
+    t.refs = 0;
+    t.used = ++c->use_clock;
     HIPCHK(c, hipMalloc((void **)&t.d, n * sizeof(int)));
     // on the context's stream, complete before `order` goes out of scope
     hipError_t e = hipMemcpyAsync(t.d, order.data(), n * sizeof(int), hipMemcpyHostToDevice,
@@ -817,6 +845,7 @@ int rt_capture_begin(rt_ctx *c)
         return rc;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
+    c->capture_orders.clear();
     return RT_OK;
 }
 
@@ -845,6 +874,10 @@ int rt_capture_end(rt_ctx *c, rt_graph **out)
         delete G;
         return fail(c, RT_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
     }
+    G->orders = c->capture_orders;
+    for (auto &t : c->orders)
+        if (std::find(G->orders.begin(), G->orders.end(), t.d) != G->orders.end())
+            t.refs++;
     *out = G;
     return RT_OK;
 }
@@ -880,6 +913,9 @@ int rt_graph_destroy(rt_ctx *c, rt_graph *g)
         (void)hipGraphExecDestroy(g->exec);
     if (g->graph)
         (void)hipGraphDestroy(g->graph);
+    for (auto &t : c->orders)
+        if (t.refs > 0 && std::find(g->orders.begin(), g->orders.end(), t.d) != g->orders.end())
+            t.refs--;
     delete g;
     return RT_OK;
 }

@@ -143,8 +143,10 @@ def test_graph_replay_matches_eager(tracer):
 
 def test_graph_keeps_its_claim_order_after_other_shapes(tracer):
     """A graph captured at one image shape replays correctly after renders at
-    other shapes have built new claim-order tables; a shape first seen during
-    a capture is refused."""
+    other shapes have built new claim-order tables -- more shapes than the
+    context keeps, so unpinned tables are evicted around the graph's pinned
+    one, and evicted shapes render identically when their table is rebuilt;
+    a shape first seen during a capture is refused."""
     W = 24
     _params(tracer, W)
     n = W * W
@@ -157,9 +159,13 @@ def test_graph_keeps_its_claim_order_after_other_shapes(tracer):
         tracer.render_device([INCL], [FREQ], d1)
         g = tracer.capture_end()
         try:
-            for W2 in (31, 64):
+            first = {}
+            for W2 in [31, 64] + list(range(10, 22)) + [31, 64, 10]:
                 _params(tracer, W2)
-                tracer.render([INCL], [FREQ])
+                img, _ = tracer.render([INCL], [FREQ])
+                if W2 in first:
+                    assert np.array_equal(img, first[W2]), W2
+                first.setdefault(W2, img)
                 g.launch()
                 tracer.synchronize()
                 assert np.array_equal(tracer.d2h(d1, (n,), np.float64), e1), W2
