@@ -148,16 +148,28 @@ double rt_jansky_factor(const rt_ctx *ctx);
  * of the tiles, RT_ORDER_CENTRE (by distance from the image centre, so the
  * long rays near the photon ring start first and the short escaping rays
  * fill the end of the launch) or RT_ORDER_SCAN (row-major); 0 = library
- * default (centre).  None of these changes any output bit.  The centre order
- * for a new image shape is built by the first eager render of that shape. */
+ * default (centre).  approach: the share of the rays, in claim order,
+ * whose camera-to-grid leg runs first in a kernel of its own: 1..100
+ * percent, RT_APPROACH_NONE for none, 0 = library default (all rays for
+ * launches of at least 2^20 rays, none below).  Those rays are
+ * saved where they reach the grid (120 bytes each, at most 2^23 saved rays
+ * per launch pair, larger batches in chunks) and resumed first by the
+ * persistent kernel, whose waves then do no camera-to-grid steps beside
+ * rays inside the grid; it traces the other rays from the camera, and
+ * these fill the time in which the longest resumed rays finish.  None of
+ * these changes any output bit.  The centre order for a new image shape, and
+ * the buffer of saved rays, are made by an eager render, so render once
+ * before capturing a graph. */
 #define RT_ORDER_CENTRE 1
 #define RT_ORDER_SCAN 2
+#define RT_APPROACH_NONE (-1)
 typedef struct {
     int block;
     int blocks_per_cu;
     int refill;
     int tile;
     int order;
+    int approach;
 } rt_launch_config;
 
 int rt_set_launch_config(rt_ctx *ctx, const rt_launch_config *cfg);

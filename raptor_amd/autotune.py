@@ -1,8 +1,10 @@
 """Autotuner for the persistent trace kernel's launch shape.
 
 Knobs (rt_launch_config): resident workgroups per CU (occupancy), the wave
-refill threshold, the T x T pixel tiles in which rays are handed out, and the
-order of the tiles (centre-out or row-major).  `tune` times every candidate on the caller's own workload (same dump,
+refill threshold, the T x T pixel tiles in which rays are handed out, the
+order of the tiles (centre-out or row-major) and the share of rays whose
+camera-to-grid leg runs in the approach kernel (all or none).  `tune` times
+every candidate on the caller's own workload (same dump,
 image size, cameras, frequencies, gradient flag) with the library's HIP-event
 kernel timer, keeps the fastest, applies it to the tracer and stores it in a
 JSON cache keyed by the workload class, so later runs start from it
@@ -16,14 +18,14 @@ import os
 
 import numpy as np
 
-from .rt import ORDER_CENTRE, ORDER_SCAN
+from .rt import APPROACH_NONE, ORDER_CENTRE, ORDER_SCAN
 
 CACHE = os.environ.get("RAPTOR_AUTOTUNE_CACHE",
                        os.path.join(os.path.expanduser("~"), ".cache", "raptor_amd",
                                     "autotune.json"))
 
-DEFAULT_SPACE = dict(blocks_per_cu=(0, 4, 6), refill=(1, 32), tile=(1, 4, 8, 16),
-                     order=(ORDER_CENTRE, ORDER_SCAN))
+DEFAULT_SPACE = dict(blocks_per_cu=(0, 4, 6), refill=(1, 4, 32), tile=(1, 4, 8, 16),
+                     order=(ORDER_CENTRE, ORDER_SCAN), approach=(APPROACH_NONE, 100))
 
 
 def workload_key(tracer, n_cam, n_freq, grad, dim3):
@@ -70,9 +72,9 @@ def tune(tracer, inclinations, freqs, grad=False, dim3=False, space=None, reps=2
     d_dimg = tracer.alloc(2 * n * 8) if grad else None
     table = []
     try:
-        for bpc, refill, tile, order in itertools.product(space["blocks_per_cu"], space["refill"],
-                                                          space["tile"], space["order"]):
-            cfg = dict(blocks_per_cu=bpc, refill=refill, tile=tile, order=order)
+        keys = sorted(space)
+        for values in itertools.product(*(space[k] for k in keys)):
+            cfg = dict(zip(keys, values))
             tracer.set_launch_config(**cfg)
             tracer.render_device(incl, fr, d_img, d_dimg)
             tracer.synchronize()

@@ -26,13 +26,44 @@ using namespace rt;
 
 static constexpr int kDefaultRefill = 4;
 static constexpr int kDefaultTile = 8;   // one wave's 64 lanes take one 8 x 8 tile
+// Approach kernel (percent of rays) by default: all rays of launches of at
+// least kApproachRays, whose persistent kernel then runs long beside its
+// longest ray; none below, where the kernel lasts about as long as its
+// longest ray anyway and the approach kernel would only add its own time.
+static constexpr long long kApproachRays = 1LL << 20;
 
 namespace {
+
+// Rays that have reached the grid, saved by the approach kernel for the trace
+// kernel to resume, each as it was before the step that took it into the
+// grid.  Slots in blocks of 64 (one wave's rays), each block a structure of
+// arrays: 13 doubles [13][64] (X^mu, k^mu, r, sin/cos theta, sin/cos 2 pi X2)
+// then 4 ints [4][64] (steps, turns, ray, flags), so a wave's store of one
+// field is 512 contiguous bytes and every field sits at a fixed offset.
+// ray < 0: nothing left to trace.  flags: theta-dot sign + 1 | (beta sign +
+// 1) << 2.
+struct RayState {
+    char *d;
+    int n;   // slots (a multiple of 64); 0: no saved state
+};
+static constexpr int kStateBlockBytes = 64 * (13 * 8 + 4 * 4);
+static constexpr size_t kStateBytes = kStateBlockBytes / 64;
+
+__device__ inline double *state_f64(const RayState &S, int i)
+{
+    return (double *)(S.d + (size_t)(i >> 6) * kStateBlockBytes) + (i & 63);
+}
+__device__ inline int *state_i32(const RayState &S, int i)
+{
+    return (int *)(S.d + (size_t)(i >> 6) * kStateBlockBytes + 13 * 64 * 8) + (i & 63);
+}
 
 struct TraceArgs {
     const float4 *prims;
     unsigned int *queue;
-    int total;          // work items: n_cam * (tiles * tile^2, or W*H)
+    int total;          // work items of this launch
+    int w0;             // index of its first work item
+    int nsaved;         // its first nsaved items were saved by the approach kernel
     int refill;
     int tile, ntx;      // work-item order: T x T pixel tiles, ntx tiles per row
     int per_cam;        // work items per camera
@@ -40,6 +71,7 @@ struct TraceArgs {
     double *image;
     double *dimage;
     rt_ray_counters *counters;
+    RayState st;        // the saved rays
 };
 
 template <int NF, bool GRAD, bool DIM3>
@@ -90,16 +122,123 @@ __device__ inline bool work_to_ray(const Params &P, const TraceArgs &A, int w, i
     return true;
 }
 
-// Claims work items until one maps to a pixel or the queue is exhausted.
-__device__ inline bool claim(const Params &P, const TraceArgs &A, int &ray)
+// Rays that end before reaching the grid (escaped or out of steps) carry no
+// emission: zero outputs.
+__device__ inline void write_empty(const Params &P, const TraceArgs &A, int ray, int steps)
+{
+    const size_t o = (size_t)ray * P.nf;
+    for (int f = 0; f < P.nf; f++)
+        A.image[o + f] = 0.;
+    if (A.dimage)
+        for (int f = 0; f < 2 * P.nf; f++)
+            A.dimage[2 * o + f] = 0.;
+    if (A.counters) {
+        rt_ray_counters c;
+        c.steps = steps;
+        c.samples = 0;
+        A.counters[ray] = c;
+    }
+}
+
+template <int NF, bool GRAD>
+__device__ inline void save_state(const RayState &S, int i, int ray, const Ray<NF, GRAD> &R)
+{
+    double *f = state_f64(S, i);
+    int *k = state_i32(S, i);
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+        f[q * 64] = R.y[q];
+    f[8 * 64] = R.pt.r;
+    f[9 * 64] = R.pt.s;
+    f[10 * 64] = R.pt.c;
+    f[11 * 64] = R.pt.s2;
+    f[12 * 64] = R.pt.c2;
+    k[0] = R.steps;
+    k[64] = R.turns;
+    k[128] = ray;
+    k[192] = (R.thetadot_sign + 1) | ((R.beta_sign + 1) << 2);
+}
+
+__device__ inline void drop_state(const RayState &S, int i) { state_i32(S, i)[128] = -1; }
+
+__device__ inline int state_ray(const RayState &S, int i) { return state_i32(S, i)[128]; }
+
+template <int NF, bool GRAD>
+__device__ inline void load_state(const RayState &S, int i, Ray<NF, GRAD> &R)
+{
+    const double *f = state_f64(S, i);
+    const int *k = state_i32(S, i);
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+        R.y[q] = f[q * 64];
+    R.pt.r = f[8 * 64];
+    R.pt.s = f[9 * 64];
+    R.pt.c = f[10 * 64];
+    R.pt.s2 = f[11 * 64];
+    R.pt.c2 = f[12 * 64];
+    R.steps = k[0];
+    R.turns = k[64];
+    const int fl = k[192];
+    R.thetadot_sign = (fl & 3) - 1;
+    R.beta_sign = ((fl >> 2) & 3) - 1;
+    R.samples = 0;
+    R.terminate = false;
+    R.acc.clear();
+}
+
+// The approach: one thread per work item traces its ray from the camera to
+// the grid (approach_ray) and leaves it saved for the trace kernel.  Every
+// ray's approach is about ln(r_cam / r_out) / STEPSIZE steps with no fluid
+// work, so these waves run without the idle lanes they would have beside rays
+// inside the grid.
+template <int INT>
+__global__ __launch_bounds__(64, 3) void approach_kernel(const Params P, const TraceArgs A)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= A.nsaved)
+        return;
+    int ray;
+    if (!work_to_ray(P, A, A.w0 + i, ray)) {
+        drop_state(A.st, i);
+        return;
+    }
+    Ray<1, false> R;
+    auto save = [&](const Ray<1, false> &S) { save_state(A.st, i, ray, S); };
+    if (approach_ray<INT>(P, ray / P.npix, ray % P.npix, P.stopx1 + 4. * P.stepsize, R, save))
+        return;
+    write_empty(P, A, ray, R.steps);
+    drop_state(A.st, i);
+}
+
+// Claims work items until one maps to a ray still to trace or the queue is
+// exhausted; `slot` is the item's index within the launch.  RESUME: the rays
+// were saved by the approach kernel.
+template <bool RESUME>
+__device__ inline bool claim(const Params &P, const TraceArgs &A, int &ray, int &slot)
 {
     while (true) {
-        const int w = (int)atomicAdd(A.queue, 1u);
-        if (w >= A.total)
+        const int q = (int)atomicAdd(A.queue, 1u);
+        if (q >= A.total)
             return false;
-        if (work_to_ray(P, A, w, ray))
+        slot = q;
+        if (RESUME && q < A.nsaved) {
+            ray = state_ray(A.st, q);
+            if (ray >= 0)
+                return true;
+        } else if (work_to_ray(P, A, A.w0 + q, ray)) {
             return true;
+        }
     }
+}
+
+template <bool RESUME, int NF, bool GRAD>
+__device__ inline void start_ray(const Params &P, const TraceArgs &A, int ray, int slot,
+                                 Ray<NF, GRAD> &R)
+{
+    if (RESUME && slot < A.nsaved)
+        load_state(A.st, slot, R);
+    else
+        ray_init<NF, GRAD>(P, ray / P.npix, ray % P.npix, R);
 }
 
 // Each lane traces one ray at a time.  A lane whose ray has ended waits until
@@ -121,21 +260,22 @@ __device__ inline bool claim(const Params &P, const TraceArgs &A, int &ray)
 // trades its instruction scheduling for registers they do not need
 // (measured slower).
 static constexpr long long kThreeWaveRays = 1LL << 19;
+static constexpr int kLongRay = 800;   // steps: about the 99.5th percentile at 1024^2
 template <int NF, bool GRAD, bool DIM3, int INT>
 constexpr bool three_waves()
 {
     return INT == kRK2 && NF == 1 && !(GRAD && DIM3);
 }
 
-template <int NF, bool GRAD, bool DIM3, int INT>
+template <int NF, bool GRAD, bool DIM3, int INT, bool RESUME>
 __device__ __forceinline__ void trace_rays(const Params &P, const TraceArgs &A)
 {
     Ray<NF, GRAD> R;
-    int ray = 0;
-    bool active = claim(P, A, ray);
+    int ray = 0, slot = 0;
+    bool active = claim<RESUME>(P, A, ray, slot);
     bool exhausted = !active;
     if (active)
-        ray_init<NF, GRAD>(P, ray / P.npix, ray % P.npix, R);
+        start_ray<RESUME>(P, A, ray, slot, R);
     while (true) {
         if (active) {
             bool cont = ray_alive<NF, GRAD>(P, R);
@@ -146,6 +286,14 @@ __device__ __forceinline__ void trace_rays(const Params &P, const TraceArgs &A)
                 active = false;
             }
         }
+        // A wave holding a ray that has run kLongRay steps (a ray near the
+        // photon ring: these orbit the hole and take up to a few thousand
+        // steps) issues ahead of the other waves on its SIMD, so that the
+        // launch does not end waiting on it.
+        if (__ballot(active && R.steps > kLongRay))
+            __builtin_amdgcn_s_setprio(2);
+        else
+            __builtin_amdgcn_s_setprio(0);
         const unsigned long long act = __ballot(active);
         exhausted = __ballot(exhausted) != 0ull;
         const int idle = 64 - __popcll(act);
@@ -154,8 +302,8 @@ __device__ __forceinline__ void trace_rays(const Params &P, const TraceArgs &A)
                 break;
         } else if (idle >= A.refill || (act == 0ull)) {
             if (!active) {
-                if (claim(P, A, ray)) {
-                    ray_init<NF, GRAD>(P, ray / P.npix, ray % P.npix, R);
+                if (claim<RESUME>(P, A, ray, slot)) {
+                    start_ray<RESUME>(P, A, ray, slot, R);
                     active = true;
                 } else {
                     exhausted = true;
@@ -165,16 +313,16 @@ __device__ __forceinline__ void trace_rays(const Params &P, const TraceArgs &A)
     }
 }
 
-template <int NF, bool GRAD, bool DIM3, int INT>
+template <int NF, bool GRAD, bool DIM3, int INT, bool RESUME>
 __global__ __launch_bounds__(64) void trace_kernel(const Params P, const TraceArgs A)
 {
-    trace_rays<NF, GRAD, DIM3, INT>(P, A);
+    trace_rays<NF, GRAD, DIM3, INT, RESUME>(P, A);
 }
 
-template <int NF, bool GRAD, bool DIM3, int INT>
+template <int NF, bool GRAD, bool DIM3, int INT, bool RESUME>
 __global__ __launch_bounds__(64, 3) void trace_kernel_3w(const Params P, const TraceArgs A)
 {
-    trace_rays<NF, GRAD, DIM3, INT>(P, A);
+    trace_rays<NF, GRAD, DIM3, INT, RESUME>(P, A);
 }
 
 __global__ void reset_queue_kernel(unsigned int *q) { *q = 0u; }
@@ -322,11 +470,19 @@ struct rt_ctx {
     };
     std::vector<OrderTable> orders;
     unsigned long long use_clock = 0;
-    std::vector<int *> capture_orders;   // tables used by the open capture
+    // Saved-ray buffers of the split launch.  back() is the current one; an
+    // older one stays only while a captured graph launches with it.
+    struct StateBuf {
+        void *d;
+        int slots;
+        int refs;
+    };
+    std::vector<StateBuf> states;
+    std::vector<void *> capture_pins;   // tables and buffers used by the open capture
     double *d_image = nullptr, *d_dimage = nullptr;
     size_t image_cap = 0, dimage_cap = 0;
     double last_ms = 0.;
-    rt_launch_config cfg{64, 0, kDefaultRefill, kDefaultTile, RT_ORDER_CENTRE};
+    rt_launch_config cfg{64, 0, kDefaultRefill, kDefaultTile, RT_ORDER_CENTRE, 0};
     bool capturing = false;   // between rt_capture_begin and rt_capture_end
     char err[512] = {0};
 };
@@ -335,10 +491,19 @@ struct rt_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int device = 0;
-    std::vector<int *> orders;   // claim-order tables pinned for the graph's launches
+    std::vector<void *> pins;   // order tables and state buffers its launches use
 };
 
 static constexpr size_t kMaxOrderTables = 8;
+// Work items per approach + trace launch pair with the split launch: 2^23
+// saved rays are 1 GB; larger batches run in chunks of this size.
+static constexpr int kMaxChunk = 1 << 23;
+
+static void pin(rt_ctx *c, void *p)
+{
+    if (std::find(c->capture_pins.begin(), c->capture_pins.end(), p) == c->capture_pins.end())
+        c->capture_pins.push_back(p);
+}
 
 static char g_create_err[512];
 
@@ -391,13 +556,13 @@ static int make_params(rt_ctx *c, const double *incl, int ncam, const double *fr
     return RT_OK;
 }
 
-template <int NF, bool GRAD, bool DIM3, int INT>
+template <int NF, bool GRAD, bool DIM3, int INT, bool RESUME>
 static hipError_t launch_trace(rt_ctx *c, const Params &P, const TraceArgs &A)
 {
-    void (*kern)(const Params, const TraceArgs) = trace_kernel<NF, GRAD, DIM3, INT>;
+    void (*kern)(const Params, const TraceArgs) = trace_kernel<NF, GRAD, DIM3, INT, RESUME>;
     if constexpr (three_waves<NF, GRAD, DIM3, INT>()) {
         if (A.total >= kThreeWaveRays)
-            kern = trace_kernel_3w<NF, GRAD, DIM3, INT>;
+            kern = trace_kernel_3w<NF, GRAD, DIM3, INT, RESUME>;
     }
     int block = 64;
     int per_cu = c->cfg.blocks_per_cu;
@@ -415,26 +580,60 @@ static hipError_t launch_trace(rt_ctx *c, const Params &P, const TraceArgs &A)
     return hipGetLastError();
 }
 
-template <bool GRAD, bool DIM3, int INT>
+template <bool GRAD, bool DIM3, int INT, bool RESUME>
 static hipError_t dispatch_nf(rt_ctx *c, const Params &P, const TraceArgs &A)
 {
     if (P.nf == 1)
-        return launch_trace<1, GRAD, DIM3, INT>(c, P, A);
+        return launch_trace<1, GRAD, DIM3, INT, RESUME>(c, P, A);
     if (P.nf <= 4)
-        return launch_trace<4, GRAD, DIM3, INT>(c, P, A);
+        return launch_trace<4, GRAD, DIM3, INT, RESUME>(c, P, A);
     if (P.nf <= 8)
-        return launch_trace<8, GRAD, DIM3, INT>(c, P, A);
-    return launch_trace<16, GRAD, DIM3, INT>(c, P, A);
+        return launch_trace<8, GRAD, DIM3, INT, RESUME>(c, P, A);
+    return launch_trace<16, GRAD, DIM3, INT, RESUME>(c, P, A);
+}
+
+// All work items in one persistent launch (pct < 0), or chunk by chunk: the
+// approach kernel over the first pct percent of the chunk's work items, then
+// the persistent kernel, which resumes those and traces the rest from the
+// camera.  A chunk saves at most A.st.n rays.
+template <bool GRAD, bool DIM3, int INT>
+static hipError_t run_launches(rt_ctx *c, const Params &P, TraceArgs A, int total, int pct)
+{
+    hipError_t e;
+    if (pct <= 0) {
+        A.w0 = 0;
+        A.total = total;
+        A.nsaved = 0;
+        hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+        return dispatch_nf<GRAD, DIM3, INT, false>(c, P, A);
+    }
+    const long long chunk =
+        std::min<long long>(kMaxChunk, (long long)A.st.n * 100 / pct);   // saves <= st.n
+    for (long long w0 = 0; w0 < total; w0 += chunk) {
+        A.w0 = (int)w0;
+        A.total = (int)std::min<long long>(chunk, total - w0);
+        A.nsaved = (int)(((long long)A.total * pct + 99) / 100);
+        hipLaunchKernelGGL(approach_kernel<INT>, dim3((unsigned)((A.nsaved + 63) / 64)),
+                           dim3(64), 0, c->stream, P, A);
+        hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+        if ((e = dispatch_nf<GRAD, DIM3, INT, true>(c, P, A)) != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
 }
 
 template <bool GRAD, bool DIM3>
-static hipError_t dispatch_int(rt_ctx *c, const Params &P, const TraceArgs &A)
+static hipError_t dispatch_int(rt_ctx *c, const Params &P, const TraceArgs &A, int total, int pct)
 {
     if (c->params.integrator == RT_INT_RK4)
-        return dispatch_nf<GRAD, DIM3, kRK4>(c, P, A);
+        return run_launches<GRAD, DIM3, kRK4>(c, P, A, total, pct);
     if (c->params.integrator == RT_INT_VERLET)
-        return dispatch_nf<GRAD, DIM3, kVerlet>(c, P, A);
-    return dispatch_nf<GRAD, DIM3, kRK2>(c, P, A);
+        return run_launches<GRAD, DIM3, kVerlet>(c, P, A, total, pct);
+    return run_launches<GRAD, DIM3, kRK2>(c, P, A, total, pct);
 }
 
 static int ensure_buffer(rt_ctx *c, double **buf, size_t *cap, size_t bytes)
@@ -447,6 +646,33 @@ static int ensure_buffer(rt_ctx *c, double **buf, size_t *cap, size_t bytes)
     *cap = 0;
     HIPCHK(c, hipMalloc((void **)buf, bytes));
     *cap = bytes;
+    return RT_OK;
+}
+
+// The saved-ray buffer, at least `slots` rays (rounded up to 64).  A capture
+// uses the current buffer as it is (in more, smaller chunks if it is short)
+// and is refused if there is none.
+static int ensure_state(rt_ctx *c, long long slots, RayState *out)
+{
+    const int want = (int)((slots + 63) / 64 * 64);
+    if (c->states.empty() || (c->states.back().slots < want && !c->capturing)) {
+        if (c->capturing)
+            return fail(c, RT_E_STATE, "the split launch's ray buffer is allocated by an eager "
+                                       "render; render once before capturing");
+        if (!c->states.empty() && c->states.back().refs == 0) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));   // launches still reading it
+            HIPCHK(c, hipFree(c->states.back().d));
+            c->states.pop_back();
+        }
+        rt_ctx::StateBuf b{nullptr, want, 0};
+        HIPCHK(c, hipMalloc(&b.d, (size_t)want * kStateBytes));
+        c->states.push_back(b);
+    }
+    const rt_ctx::StateBuf &b = c->states.back();
+    if (c->capturing)
+        pin(c, b.d);
+    out->d = (char *)b.d;
+    out->n = b.slots;
     return RT_OK;
 }
 
@@ -466,9 +692,8 @@ static int ensure_order(rt_ctx *c, int W, int H, int T, double sx, double sy, co
     for (auto &t : c->orders) {
         if (memcmp(key, t.key, sizeof key) == 0) {
             t.used = ++c->use_clock;
-            if (c->capturing && std::find(c->capture_orders.begin(), c->capture_orders.end(),
-                                          t.d) == c->capture_orders.end())
-                c->capture_orders.push_back(t.d);
+            if (c->capturing)
+                pin(c, t.d);
             *out = t.d;
             return RT_OK;
         }
@@ -583,20 +808,26 @@ static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *fr
         A.ntx = 0;
         A.per_cam = P.npix;
     }
-    A.total = ncam * A.per_cam;
+    const int total = ncam * A.per_cam;
     A.image = d_image;
     A.dimage = grad ? d_dimage : nullptr;
     A.counters = d_counters;
-    hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue);
-    HIPCHK(c, hipGetLastError());
+    A.st = RayState{};
+    const int pct = c->cfg.approach != 0 ? c->cfg.approach
+                                         : (total >= kApproachRays ? 100 : RT_APPROACH_NONE);
+    if (pct > 0 &&
+        (rc = ensure_state(c, (std::min<long long>(total, kMaxChunk) * pct + 99) / 100, &A.st)))
+        return rc;
     if (!c->capturing)
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     hipError_t e;
     bool d3 = c->grid.n3 > 1;
     if (grad)
-        e = d3 ? dispatch_int<true, true>(c, P, A) : dispatch_int<true, false>(c, P, A);
+        e = d3 ? dispatch_int<true, true>(c, P, A, total, pct)
+               : dispatch_int<true, false>(c, P, A, total, pct);
     else
-        e = d3 ? dispatch_int<false, true>(c, P, A) : dispatch_int<false, false>(c, P, A);
+        e = d3 ? dispatch_int<false, true>(c, P, A, total, pct)
+               : dispatch_int<false, false>(c, P, A, total, pct);
     HIPCHK(c, e);
     if (!c->capturing)
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -705,6 +936,8 @@ void rt_destroy(rt_ctx *c)
     (void)hipFree(c->d_queue);
     for (auto &t : c->orders)
         (void)hipFree(t.d);
+    for (auto &b : c->states)
+        (void)hipFree(b.d);
     (void)hipFree(c->d_image);
     (void)hipFree(c->d_dimage);
     if (c->ev0)
@@ -845,7 +1078,7 @@ int rt_capture_begin(rt_ctx *c)
         return rc;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
-    c->capture_orders.clear();
+    c->capture_pins.clear();
     return RT_OK;
 }
 
@@ -874,10 +1107,16 @@ int rt_capture_end(rt_ctx *c, rt_graph **out)
         delete G;
         return fail(c, RT_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
     }
-    G->orders = c->capture_orders;
+    G->pins = c->capture_pins;
+    auto pinned = [&](void *p) {
+        return std::find(G->pins.begin(), G->pins.end(), p) != G->pins.end();
+    };
     for (auto &t : c->orders)
-        if (std::find(G->orders.begin(), G->orders.end(), t.d) != G->orders.end())
+        if (pinned(t.d))
             t.refs++;
+    for (auto &b : c->states)
+        if (pinned(b.d))
+            b.refs++;
     *out = G;
     return RT_OK;
 }
@@ -913,9 +1152,24 @@ int rt_graph_destroy(rt_ctx *c, rt_graph *g)
         (void)hipGraphExecDestroy(g->exec);
     if (g->graph)
         (void)hipGraphDestroy(g->graph);
+    auto pinned = [&](void *p) {
+        return std::find(g->pins.begin(), g->pins.end(), p) != g->pins.end();
+    };
     for (auto &t : c->orders)
-        if (t.refs > 0 && std::find(g->orders.begin(), g->orders.end(), t.d) != g->orders.end())
+        if (t.refs > 0 && pinned(t.d))
             t.refs--;
+    for (auto &b : c->states)
+        if (b.refs > 0 && pinned(b.d))
+            b.refs--;
+    // older state buffers that no graph uses any more (the stream is idle)
+    for (size_t i = 0; i + 1 < c->states.size();) {
+        if (c->states[i].refs == 0) {
+            (void)hipFree(c->states[i].d);
+            c->states.erase(c->states.begin() + (long)i);
+        } else {
+            i++;
+        }
+    }
     delete g;
     return RT_OK;
 }
@@ -926,16 +1180,18 @@ int rt_set_launch_config(rt_ctx *c, const rt_launch_config *cfg)
         return RT_E_INVALID;
     if (!cfg || (cfg->block != 0 && cfg->block != 64) || cfg->blocks_per_cu < 0 ||
         cfg->blocks_per_cu > 64 || cfg->refill < 0 || cfg->refill > 64 || cfg->tile < 0 ||
-        cfg->tile > 64 || (cfg->order != 0 && cfg->order != RT_ORDER_CENTRE &&
-                           cfg->order != RT_ORDER_SCAN))
+        cfg->tile > 64 ||
+        (cfg->order != 0 && cfg->order != RT_ORDER_CENTRE && cfg->order != RT_ORDER_SCAN) ||
+        cfg->approach < RT_APPROACH_NONE || cfg->approach > 100)
         return fail(c, RT_E_INVALID, "bad launch config (block 64, blocks_per_cu 0..64, "
                                      "refill 0..64, tile 0..64, order 0, RT_ORDER_CENTRE or "
-                                     "RT_ORDER_SCAN)");
+                                     "RT_ORDER_SCAN, approach RT_APPROACH_NONE or 0..100)");
     c->cfg.block = 64;
     c->cfg.blocks_per_cu = cfg->blocks_per_cu;
     c->cfg.refill = cfg->refill == 0 ? kDefaultRefill : cfg->refill;
     c->cfg.tile = cfg->tile == 0 ? kDefaultTile : cfg->tile;
     c->cfg.order = cfg->order == 0 ? RT_ORDER_CENTRE : cfg->order;
+    c->cfg.approach = cfg->approach;
     return RT_OK;
 }
 

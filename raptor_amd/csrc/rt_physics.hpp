@@ -831,12 +831,13 @@ __host__ __device__ inline bool ray_alive(const Params &P, const Ray<NF, GRAD> &
            !R.terminate;
 }
 
-// One iteration of integrate_geodesic's loop (integrator.c:243-284).
-// Returns false when the ray escaped through the outer grid edge (the
-// `break` at integrator.c:271-272).
-template <int NF, bool GRAD, bool DIM3, int INT = kRK2>
-__host__ __device__ inline bool ray_step(const Params &P, const float4 *__restrict__ prims,
-                                         Ray<NF, GRAD> &R)
+// The geodesic half of one iteration of integrate_geodesic's loop
+// (integrator.c:243-272): turning-point count, step size, one step of the
+// chosen integrator and the escape test.  Returns false when the ray escaped
+// through the outer grid edge (the `break` at integrator.c:271-272); `dl` is
+// the step taken, which the sample of the new point uses.
+template <int INT, int NF, bool GRAD>
+__host__ __device__ inline bool ray_advance(const Params &P, Ray<NF, GRAD> &R, double &dl)
 {
     // integrator.c:243-247: a sign change of k^theta is a turning point
     const int sgn = (R.y[6] > 0.) - (R.y[6] < 0.);
@@ -845,11 +846,24 @@ __host__ __device__ inline bool ray_step(const Params &P, const float4 *__restri
     R.thetadot_sign = sgn;
     if ((R.beta_sign < 0 && R.turns > kMaxOrder) || (R.beta_sign > 0 && R.turns > kMaxOrder + 1))
         R.terminate = true;
-    double dl = stepsize(R.y, P.stepsize);
+    dl = stepsize(R.y, P.stepsize);
     geodesic_step<INT>(R.y, R.pt, dl, P.a, P.hslope);
-    if (R.y[1] > P.stopx1 && R.y[5] < 0.)
-        return false;
-    if (R.y[1] < P.stopx1 && R.y[1] > P.startx1 && R.y[2] > P.startx2 && R.y[2] < P.stopx2) {
+    return !(R.y[1] > P.stopx1 && R.y[5] < 0.);
+}
+
+// The transfer gate of integrator.c:277: open intervals in X1 and X2.
+__host__ __device__ inline bool in_grid(const Params &P, const double y[8])
+{
+    return y[1] < P.stopx1 && y[1] > P.startx1 && y[2] > P.startx2 && y[2] < P.stopx2;
+}
+
+// The rest of the iteration (integrator.c:277-284): the sample of the new
+// point, when it is inside the grid, with the step's |dlambda|.
+template <int NF, bool GRAD, bool DIM3>
+__host__ __device__ inline void ray_sample(const Params &P, const float4 *__restrict__ prims,
+                                           Ray<NF, GRAD> &R, double dl)
+{
+    if (in_grid(P, R.y)) {
         Fluid F;
         if (fluid_sample<DIM3>(P, prims, R.pt, R.y, F)) {
             R.samples++;
@@ -857,7 +871,59 @@ __host__ __device__ inline bool ray_step(const Params &P, const float4 *__restri
         }
     }
     R.steps++;
+}
+
+// One iteration of integrate_geodesic's loop (integrator.c:243-284).
+// Returns false when the ray escaped through the outer grid edge.
+template <int NF, bool GRAD, bool DIM3, int INT = kRK2>
+__host__ __device__ inline bool ray_step(const Params &P, const float4 *__restrict__ prims,
+                                         Ray<NF, GRAD> &R)
+{
+    double dl;
+    if (!ray_advance<INT>(P, R, dl))
+        return false;
+    ray_sample<NF, GRAD, DIM3>(P, prims, R, dl);
     return true;
+}
+
+// The camera-to-grid leg of a ray (integrate_geodesic, integrator.c:228-277,
+// up to the first step that lands inside the grid): no sample happens before
+// that step.  Returns true when the ray reached the grid; `save(R)` was then
+// last called with the ray as it was before that step, so that resuming from
+// the saved ray repeats the step with the same arithmetic and goes on.
+// Returns false when the ray ended first (escaped or out of steps; R as the
+// loop left it).  The ray is saved only before steps that start at X1 <=
+// `near`: with near = stopx1 + 4 STEPSIZE (a first-order step moves X1 by at
+// most STEPSIZE) these are the last few steps of the leg.  Should a step from
+// farther out land inside the grid, the ray is traced again from the camera
+// and saved before every step.
+template <int INT, class Save>
+__host__ __device__ inline bool approach_ray(const Params &P, int cam, int pix, double near,
+                                             Ray<1, false> &R, Save &&save)
+{
+    for (int pass = 0; pass < 2; pass++) {
+        ray_init<1, false>(P, cam, pix, R);
+        bool missed = false;
+#pragma clang loop unroll(disable)
+        while (ray_alive<1, false>(P, R)) {
+            const bool saved = pass == 1 || R.y[1] <= near;
+            if (saved)
+                save(R);
+            double dl;
+            if (!ray_advance<INT>(P, R, dl))
+                break;
+            if (in_grid(P, R.y)) {
+                if (saved)
+                    return true;
+                missed = true;
+                break;
+            }
+            R.steps++;
+        }
+        if (!missed)
+            return false;
+    }
+    return false;
 }
 
 // Host set-up.

@@ -45,7 +45,8 @@ class RtParams(ctypes.Structure):
 
 class RtLaunchConfig(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
-                ("refill", ctypes.c_int), ("tile", ctypes.c_int), ("order", ctypes.c_int)]
+                ("refill", ctypes.c_int), ("tile", ctypes.c_int), ("order", ctypes.c_int),
+                ("approach", ctypes.c_int)]
 
 
 # rt_params.integrator (include/raptor_rt.h): int_method of parameters.h:186-189
@@ -54,6 +55,10 @@ INT_RK2, INT_RK4, INT_VERLET = 0, 1, 2
 # claim orders of rt_launch_config.order (include/raptor_rt.h)
 ORDER_CENTRE = 1
 ORDER_SCAN = 2
+
+# rt_launch_config.approach: percent of the rays whose camera-to-grid leg runs
+# in a kernel of its own (1..100), or none
+APPROACH_NONE = -1
 
 
 class RtError(RuntimeError):
@@ -184,17 +189,20 @@ class Tracer:
         self._check(self.lib.rt_load_prims(self.ctx, ctypes.byref(grid),
                                            prims.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
 
-    def set_launch_config(self, blocks_per_cu=0, refill=0, tile=0, order=0):
-        """order: ORDER_CENTRE, ORDER_SCAN or 0 (library default, centre)."""
+    def set_launch_config(self, blocks_per_cu=0, refill=0, tile=0, order=0, approach=0):
+        """order: ORDER_CENTRE, ORDER_SCAN or 0 (library default, centre);
+        approach: percent of the rays (in claim order) whose camera-to-grid leg
+        runs in a kernel of its own, 1..100, APPROACH_NONE or 0 (library
+        default)."""
         cfg = RtLaunchConfig(block=64, blocks_per_cu=blocks_per_cu, refill=refill, tile=tile,
-                             order=order)
+                             order=order, approach=approach)
         self._check(self.lib.rt_set_launch_config(self.ctx, ctypes.byref(cfg)))
 
     def get_launch_config(self):
         cfg = RtLaunchConfig()
         self._check(self.lib.rt_get_launch_config(self.ctx, ctypes.byref(cfg)))
         return dict(block=cfg.block, blocks_per_cu=cfg.blocks_per_cu, refill=cfg.refill,
-                    tile=cfg.tile, order=cfg.order)
+                    tile=cfg.tile, order=cfg.order, approach=cfg.approach)
 
     # -- the pixel loop -----------------------------------------------------------
     def render(self, inclinations, freqs, grad=False):

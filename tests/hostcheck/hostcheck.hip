@@ -27,36 +27,87 @@ static inline __host__ void sincospi(double x, double *s, double *c)
 using namespace rt;
 
 template <int NF, bool GRAD, int INT>
-static void run(const Params &P, const float4 *prims, int total, double *img, double *dimg,
-                int *cnt)
+static void finish(const Params &P, const float4 *prims, int ray, Ray<NF, GRAD> &R, double *img,
+                   double *dimg, int *cnt)
+{
+    const bool dim3 = P.n3 > 1;
+    while (ray_alive<NF, GRAD>(P, R)) {
+        bool cont = dim3 ? ray_step<NF, GRAD, true, INT>(P, prims, R)
+                         : ray_step<NF, GRAD, false, INT>(P, prims, R);
+        if (!cont)
+            break;
+    }
+    for (int f = 0; f < P.nf; f++) {
+        img[(size_t)ray * P.nf + f] = R.acc.I[f];
+        if (GRAD) {
+            dimg[((size_t)ray * P.nf + f) * 2] = R.acc.dI[f][0];
+            dimg[((size_t)ray * P.nf + f) * 2 + 1] = R.acc.dI[f][1];
+        }
+    }
+    if (cnt) {
+        cnt[2 * ray] = R.steps;
+        cnt[2 * ray + 1] = R.samples;
+    }
+}
+
+// split == 0: every ray traced in one loop from the camera (the fused
+// launch); otherwise the split launch's path: approach_ray with near =
+// stopx1 + near_off, the saved ray copied field by field as the trace kernel
+// loads it, then the same loop.
+template <int NF, bool GRAD, int INT>
+static void run(const Params &P, const float4 *prims, int total, int split, double near_off,
+                double *img, double *dimg, int *cnt)
 {
     for (int ray = 0; ray < total; ray++) {
         Ray<NF, GRAD> R;
-        ray_init<NF, GRAD>(P, ray / P.npix, ray % P.npix, R);
-        bool dim3 = P.n3 > 1;
-        while (ray_alive<NF, GRAD>(P, R)) {
-            bool cont = dim3 ? ray_step<NF, GRAD, true, INT>(P, prims, R)
-                             : ray_step<NF, GRAD, false, INT>(P, prims, R);
-            if (!cont)
-                break;
+        if (!split) {
+            ray_init<NF, GRAD>(P, ray / P.npix, ray % P.npix, R);
+            finish<NF, GRAD, INT>(P, prims, ray, R, img, dimg, cnt);
+            continue;
         }
-        for (int f = 0; f < P.nf; f++) {
-            img[(size_t)ray * P.nf + f] = R.acc.I[f];
-            if (GRAD) {
-                dimg[((size_t)ray * P.nf + f) * 2] = R.acc.dI[f][0];
-                dimg[((size_t)ray * P.nf + f) * 2 + 1] = R.acc.dI[f][1];
+        Ray<1, false> A, S;
+        bool reached = approach_ray<INT>(P, ray / P.npix, ray % P.npix, P.stopx1 + near_off, A,
+                                         [&](const Ray<1, false> &r) { S = r; });
+        if (!reached) {
+            for (int f = 0; f < P.nf; f++) {
+                img[(size_t)ray * P.nf + f] = 0.;
+                if (GRAD)
+                    dimg[((size_t)ray * P.nf + f) * 2] = dimg[((size_t)ray * P.nf + f) * 2 + 1] = 0.;
             }
+            if (cnt) {
+                cnt[2 * ray] = A.steps;
+                cnt[2 * ray + 1] = 0;
+            }
+            continue;
         }
-        if (cnt) {
-            cnt[2 * ray] = R.steps;
-            cnt[2 * ray + 1] = R.samples;
-        }
+        for (int q = 0; q < 8; q++)
+            R.y[q] = S.y[q];
+        R.pt = S.pt;
+        R.steps = S.steps;
+        R.turns = S.turns;
+        R.thetadot_sign = S.thetadot_sign;
+        R.beta_sign = S.beta_sign;
+        R.samples = 0;
+        R.terminate = false;
+        R.acc.clear();
+        finish<NF, GRAD, INT>(P, prims, ray, R, img, dimg, cnt);
     }
 }
+
+extern "C" int hc_render_split(const rt_params *p, const rt_grid *g, const float *soa,
+                               const double *incl, int ncam, const double *freqs, int nf,
+                               int split, double near_off, double *img, double *dimg, int *cnt);
 
 extern "C" int hc_render(const rt_params *p, const rt_grid *g, const float *soa,
                          const double *incl, int ncam, const double *freqs, int nf, double *img,
                          double *dimg, int *cnt)
+{
+    return hc_render_split(p, g, soa, incl, ncam, freqs, nf, 0, 0., img, dimg, cnt);
+}
+
+extern "C" int hc_render_split(const rt_params *p, const rt_grid *g, const float *soa,
+                               const double *incl, int ncam, const double *freqs, int nf,
+                               int split, double near_off, double *img, double *dimg, int *cnt)
 {
     Params *P = new Params;
     const char *msg = fill_params(*p, *g, incl, ncam, freqs, nf, *P);
@@ -75,11 +126,11 @@ extern "C" int hc_render(const rt_params *p, const rt_grid *g, const float *soa,
     auto go = [&](auto integ) {
         constexpr int INT = decltype(integ)::value;
         if (nf == 1) {
-            if (dimg) run<1, true, INT>(*P, aos.data(), total, img, dimg, cnt);
-            else run<1, false, INT>(*P, aos.data(), total, img, dimg, cnt);
+            if (dimg) run<1, true, INT>(*P, aos.data(), total, split, near_off, img, dimg, cnt);
+            else run<1, false, INT>(*P, aos.data(), total, split, near_off, img, dimg, cnt);
         } else {
-            if (dimg) run<16, true, INT>(*P, aos.data(), total, img, dimg, cnt);
-            else run<16, false, INT>(*P, aos.data(), total, img, dimg, cnt);
+            if (dimg) run<16, true, INT>(*P, aos.data(), total, split, near_off, img, dimg, cnt);
+            else run<16, false, INT>(*P, aos.data(), total, split, near_off, img, dimg, cnt);
         }
     };
     if (p->integrator == RT_INT_RK4)

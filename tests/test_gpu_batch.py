@@ -33,7 +33,12 @@ def _params(tr, W, H=None):
                                  dict(tile=1), dict(order=rt.ORDER_SCAN, tile=1),
                                  dict(order=rt.ORDER_SCAN), dict(order=rt.ORDER_SCAN, tile=8),
                                  dict(order=rt.ORDER_SCAN, tile=5, refill=9),
-                                 dict(order=rt.ORDER_CENTRE, tile=64)])
+                                 dict(order=rt.ORDER_CENTRE, tile=64),
+                                 dict(approach=rt.APPROACH_NONE),
+                                 dict(approach=rt.APPROACH_NONE, order=rt.ORDER_SCAN, tile=5),
+                                 dict(approach=100, refill=1, tile=1),
+                                 dict(approach=1, order=rt.ORDER_SCAN, tile=5, refill=9),
+                                 dict(approach=73, tile=3)])
 def test_launch_shapes_are_bit_identical(tracer, cfg):
     _params(tracer, 45, 37)                      # ragged against every tile size
     tracer.set_launch_config()
@@ -53,7 +58,89 @@ def test_bad_launch_config_rejected(tracer):
         tracer.set_launch_config(tile=-1)
     with pytest.raises(rt.RtError):
         tracer.set_launch_config(order=3)
-    assert tracer.get_launch_config()["order"] == rt.ORDER_CENTRE
+    with pytest.raises(rt.RtError):
+        tracer.set_launch_config(approach=101)
+    with pytest.raises(rt.RtError):
+        tracer.set_launch_config(approach=-2)
+    cfg = tracer.get_launch_config()
+    assert cfg["order"] == rt.ORDER_CENTRE and cfg["approach"] == 0
+
+
+def test_split_equals_fused_with_gradients_and_counters(tracer):
+    """Launches with an approach kernel (all, some or none of the rays saved
+    where they reach the grid and resumed by the trace kernel) give the same
+    image, gradient and per-ray counters, for one and several frequencies."""
+    W = 48
+    _params(tracer, W)
+    n = 2 * W * W
+    d = tracer.alloc(n * 3 * 8)
+    dd = tracer.alloc(n * 3 * 16)
+    dc = tracer.alloc(n * 8)
+    try:
+        for freqs in ([FREQ], [FREQ, 86e9, 345e9]):
+            out = []
+            for approach in (rt.APPROACH_NONE, 100, 50, 7):
+                tracer.set_launch_config(approach=approach)
+                tracer.render_device([INCL, 20.0], freqs, d, dd, dc)
+                tracer.synchronize()
+                nf = len(freqs)
+                out.append((tracer.d2h(d, (n, nf), np.float64),
+                            tracer.d2h(dd, (n, nf, 2), np.float64),
+                            tracer.d2h(dc, (n, 2), np.int32)))
+            for o in out[1:]:
+                for a, b in zip(out[0], o):
+                    assert np.array_equal(a, b), len(freqs)
+            assert out[0][0].max() > 0 and (out[0][2][:, 1] > 0).any()
+    finally:
+        tracer.set_launch_config()
+        for p in (d, dd, dc):
+            tracer.free(p)
+
+
+def test_split_capture_chunks_and_pins_its_ray_buffer(dump128):
+    """The buffer of saved rays is sized by eager renders.  A capture larger
+    than the buffer runs in chunks that fit it; a later eager render that
+    grows the buffer leaves the graph's one in place until the graph is
+    destroyed; a capture before any eager render is refused."""
+    tr = rt.Tracer(0)
+    try:
+        tr.load_dump(dump128)
+        W = 40
+        tr.set_params(width=W, height=W, M_unit=M_UNIT, R_high=R_HIGH)
+        cams = [INCL, 30.0, 75.0]
+        n = len(cams) * W * W
+        d = tr.alloc(n * 8)
+        try:
+            tr.set_launch_config(order=rt.ORDER_SCAN, approach=50)
+            tr.capture_begin()
+            try:
+                with pytest.raises(rt.RtError, match="eager"):
+                    tr.render_device([INCL], [FREQ], d)
+            finally:
+                tr.capture_end().destroy()
+            tr.set_launch_config(approach=50)
+            one, _ = tr.render([INCL], [FREQ])      # buffer: half a camera's rays
+            tr.capture_begin()
+            tr.render_device(cams, [FREQ], d)       # chunks of at most one camera
+            g = tr.capture_end()
+            try:
+                three, _ = tr.render(cams, [FREQ])  # grows the buffer
+                assert np.array_equal(three[0], one[0])
+                for _ in range(2):
+                    tr.render_device([5.0, 15.0, 85.0], [FREQ], d)   # other values in d
+                    tr.synchronize()
+                    g.launch()
+                    tr.synchronize()
+                    got = tr.d2h(d, (n,), np.float64).reshape(len(cams), W * W)
+                    assert np.array_equal(got, three[:, :, 0])
+            finally:
+                g.destroy()
+            again, _ = tr.render(cams, [FREQ])
+            assert np.array_equal(again, three)
+        finally:
+            tr.free(d)
+    finally:
+        tr.close()
 
 
 def test_centre_order_covers_every_pixel_once(tracer):
@@ -199,7 +286,8 @@ def test_autotune_picks_a_valid_config_and_caches_it(tracer, tmp_path):
     path = str(tmp_path / "tune.json")
     res = autotune.tune(tracer, [INCL], [FREQ], space=dict(blocks_per_cu=(0, 4),
                                                            refill=(1,), tile=(1, 8),
-                                                           order=(rt.ORDER_CENTRE,)),
+                                                           order=(rt.ORDER_CENTRE,),
+                                                           approach=(100,)),
                         reps=1, path=path)
     assert len(res["table"]) == 4 and res["ms"] > 0
     assert tracer.get_launch_config()["tile"] == res["config"]["tile"]

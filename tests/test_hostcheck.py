@@ -31,7 +31,8 @@ def _lib():
     return ctypes.CDLL(LIB)
 
 
-def _render(dump, W, incl, freqs, grad=False, R_high=R_HIGH, stepsize=0.03, integrator=0):
+def _render(dump, W, incl, freqs, grad=False, R_high=R_HIGH, stepsize=0.03, integrator=0,
+            near_off=None):
     L = _lib()
     g = dump.grid
     grid = rt.RtGrid(n1=g.n1, n2=g.n2, n3=g.n3, startx1=g.startx1, startx2=g.startx2,
@@ -48,10 +49,16 @@ def _render(dump, W, incl, freqs, grad=False, R_high=R_HIGH, stepsize=0.03, inte
     dimg = np.zeros(2 * n * fr.size) if grad else None
     cnt = np.zeros(2 * n, np.int32)
     vp = ctypes.c_void_p
-    rc = L.hc_render(ctypes.byref(p), ctypes.byref(grid), vp(prims.ctypes.data),
-                     vp(incl.ctypes.data), incl.size, vp(fr.ctypes.data), fr.size,
-                     vp(img.ctypes.data), vp(dimg.ctypes.data) if grad else None,
-                     vp(cnt.ctypes.data))
+    if near_off is None:
+        rc = L.hc_render(ctypes.byref(p), ctypes.byref(grid), vp(prims.ctypes.data),
+                         vp(incl.ctypes.data), incl.size, vp(fr.ctypes.data), fr.size,
+                         vp(img.ctypes.data), vp(dimg.ctypes.data) if grad else None,
+                         vp(cnt.ctypes.data))
+    else:
+        rc = L.hc_render_split(ctypes.byref(p), ctypes.byref(grid), vp(prims.ctypes.data),
+                               vp(incl.ctypes.data), incl.size, vp(fr.ctypes.data), fr.size,
+                               1, ctypes.c_double(near_off), vp(img.ctypes.data),
+                               vp(dimg.ctypes.data) if grad else None, vp(cnt.ctypes.data))
     assert rc == 0
     return img.reshape(n, fr.size), dimg, cnt.reshape(n, 2)
 
@@ -146,3 +153,25 @@ def test_host_build_other_integrators(dump128, integrator):
                   R_high=R_HIGH)
     rk2 = O.render(P2, dump128.prims, [FREQ])
     assert np.abs(rk2[:, 0] - oimg[:, 0]).max() > 1e-6 * np.abs(oimg).max()
+
+
+@pytest.mark.parametrize("integrator", [O.RK2, O.RK4, O.VERLET])
+def test_split_approach_equals_one_loop(dump128, integrator):
+    """The split launch's path (approach_ray to the grid, the ray saved before
+    its first step inside and resumed from there) gives the same bits as one
+    loop from the camera: with the kernel's save margin (4 STEPSIZE), with a
+    margin of 0 and 1 STEPSIZE, and with a negative margin below every X1,
+    which forces the fallback that traces the ray again from the camera and
+    saves before every step."""
+    W = 16
+    stepsize = 0.03
+    ref = _render(dump128, W, [INCL, 10.0], [FREQ], grad=True, integrator=integrator)
+    for off in (4 * stepsize, stepsize, 0.0, 1e-300):
+        got = _render(dump128, W, [INCL, 10.0], [FREQ], grad=True, integrator=integrator,
+                      near_off=off)
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b), off
+    forced = _render(dump128, W, [INCL, 10.0], [FREQ], grad=True, integrator=integrator,
+                     near_off=-100.0)
+    for a, b in zip(forced, ref):
+        assert np.array_equal(a, b)
