@@ -59,6 +59,10 @@ def main():
                     help="pixels of the workload traced by the CPU oracle (default: all "
                          "of the 1024^2 image, about 10 s on 16 cores; 0 = skip)")
     ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--emission", choices=["thsynch", "thsynchav", "kappa"], default="thsynch",
+                    help="emission model of the c1 image (default: thsynch, the path's); "
+                         "the others have no gradient leg and no roofline (the frozen "
+                         "operation counts are the thermal fit's)")
     ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4"], default="c1",
                     help="c1: the headline 1024^2 image (default); c2: 6 inclinations x 8 "
                          "frequencies at 512^2, one launch vs 48; c3: 3D 192x96x96 dump at "
@@ -81,10 +85,13 @@ def main():
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
         dump = torus.fishbone_moncrief(256, 256)
+    emission = {"thsynch": rt.EMIS_THSYNCH, "thsynchav": rt.EMIS_THSYNCHAV,
+                "kappa": rt.EMIS_KAPPA}[args.emission]
+    thermal = emission == rt.EMIS_THSYNCH
     tr = rt.Tracer(0)
     tr.set_params(width=W, height=W, M_unit=WORKLOAD["M_unit"], R_high=WORKLOAD["R_high"],
                   R_low=WORKLOAD["R_low"], stepsize=WORKLOAD["stepsize"],
-                  cam_size=WORKLOAD["fov_M"])
+                  cam_size=WORKLOAD["fov_M"], emission=emission)
     tr.load_dump(dump)
     incl = [WORKLOAD["inclination_deg"]]
     freq = [WORKLOAD["freq_hz"]]
@@ -118,17 +125,23 @@ def main():
     kern_ms = float(np.mean(kms))
     achieved = alg_flops / (kern_ms * 1e-3) / 1e12
 
-    # gradient leg: image + its two parameter derivatives
-    tr.render_device(incl, freq, d_img, d_dimg)
-    tr.synchronize()
-    tg0 = time.perf_counter()
-    gms = []
-    for _ in range(args.grad_steps):
+    # gradient leg: image + its two parameter derivatives (thermal fit only)
+    gms = [float("nan")]
+    grad_rate = float("nan")
+    if thermal and args.grad_steps > 0:
         tr.render_device(incl, freq, d_img, d_dimg)
         tr.synchronize()
-        gms.append(tr.last_kernel_ms())
-    tg1 = time.perf_counter()
-    grad_rate = args.grad_steps / (tg1 - tg0)
+        tg0 = time.perf_counter()
+        gms = []
+        for _ in range(args.grad_steps):
+            tr.render_device(incl, freq, d_img, d_dimg)
+            tr.synchronize()
+            gms.append(tr.last_kernel_ms())
+        tg1 = time.perf_counter()
+        grad_rate = args.grad_steps / (tg1 - tg0)
+        # the image the parity check reads is the plain render's
+        tr.render_device(incl, freq, d_img)
+        tr.synchronize()
     img = tr.d2h(d_img, (npix,), np.float64)
     jy = tr.jansky_factor()
 
@@ -157,7 +170,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64+f32",
         "data": "synthetic (seeded Fishbone-Moncrief torus, HARM MKS, a=0.9375)",
-        "config": dict(WORKLOAD, image="%dx%d" % (W, W)),
+        "config": dict(WORKLOAD, image="%dx%d" % (W, W), emission=args.emission),
         "launch": dict(launch, approach_effective=100 if split else -1),
         "images_per_s": 1e3 / ms_per_step,
         "flux_jy": float(img.sum() * jy),
@@ -190,6 +203,11 @@ def main():
             "achieved_tflops": alg_flops_grad / (np.mean(gms) * 1e-3) / 1e12,
         },
     }
+    if not thermal:
+        result["roofline"] = None
+        result["roofline_note"] = ("no roofline: the frozen operation counts "
+                                   "(raptor_amd/flops.py) are the thermal fit's")
+        result["grad"] = None
 
     if args.cpu_sample > 0:
         from oracle import oracle as O
@@ -199,7 +217,7 @@ def main():
         P = O.params(dump.grid, width=W, height=W, inclination=incl[0],
                      M_unit=WORKLOAD["M_unit"], R_high=WORKLOAD["R_high"],
                      R_low=WORKLOAD["R_low"], cam_size=WORKLOAD["fov_M"],
-                     stepsize=WORKLOAD["stepsize"])
+                     stepsize=WORKLOAD["stepsize"], emission=emission)
         cores = host_cores()
         c0 = time.perf_counter()
         ora, gate = O.render(P, dump.prims, freq, pixels=pix, gate=True, nthreads=cores)

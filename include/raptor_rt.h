@@ -70,11 +70,21 @@ typedef struct {
     int integrator;         /* int_method (parameters.h:186-189): RT_INT_RK2
                                (0, the reference's default), RT_INT_RK4,
                                RT_INT_VERLET (integrator.c:13-105)            */
+    int emission;           /* emission model: RT_EMIS_THSYNCH (0, the one
+                               emission_coeff calls, j_nu.c:20-26),
+                               RT_EMIS_THSYNCHAV (j_nu.c:157-167 times e^2/c,
+                               Kirchhoff absorptivity) or RT_EMIS_KAPPA (the
+                               kappa = 3.5 fits of j_nu.c:28-106, width
+                               Theta_e).  Gradients: RT_EMIS_THSYNCH only.    */
 } rt_params;
 
 #define RT_INT_RK2 0
 #define RT_INT_RK4 1
 #define RT_INT_VERLET 2
+
+#define RT_EMIS_THSYNCH 0
+#define RT_EMIS_THSYNCHAV 1
+#define RT_EMIS_KAPPA 2
 
 /* Per-ray counters written when requested: RK2 steps taken and samples for
  * which the transfer was evaluated (inside the grid and past the sigma cut). */
@@ -208,12 +218,15 @@ int rt_fluid_params(rt_ctx *ctx, const double *h_X, int n, double *h_out);
  *   RT_PROBE_RK2        in y[8]          -> out dl, y'[8] (integrator.c:44-70, 109-122)
  *   RT_PROBE_CAMERA     in alpha, beta, inclination -> out y[8]     (metric.c:650-727)
  *   RT_PROBE_EMISSION   in B, cos(pitch), Thetae, nu_p, Ne -> out j, alpha_nu (j_nu.c:130-200)
+ *   RT_PROBE_EMISSION_MODEL  in B, cos(pitch), Thetae, nu_p, Ne, model (RT_EMIS_*)
+ *                            -> out j, alpha_nu of that model (j_nu.c:28-200)
  */
 #define RT_PROBE_METRIC 1
 #define RT_PROBE_ACCEL 2
 #define RT_PROBE_RK2 3
 #define RT_PROBE_CAMERA 4
 #define RT_PROBE_EMISSION 5
+#define RT_PROBE_EMISSION_MODEL 6
 int rt_probe(rt_ctx *ctx, int kind, const double *h_in, int n, double *h_out);
 
 #ifdef __cplusplus

@@ -27,12 +27,15 @@ class OrParams(ctypes.Structure):
         ("R_high", ctypes.c_double), ("R_low", ctypes.c_double),
         ("cam_size_x", ctypes.c_double), ("cam_size_y", ctypes.c_double),
         ("inclination", ctypes.c_double), ("stepsize", ctypes.c_double),
-        ("integrator", ctypes.c_int),
+        ("integrator", ctypes.c_int), ("emission", ctypes.c_int),
     ]
 
 
 # int_method of parameters.h:186-189
 RK2, RK4, VERLET = 0, 1, 2
+# emission models (or_emission): the thermal fit of the path, the angle-averaged
+# thermal fit and the kappa fits of j_nu.c
+THSYNCH, THSYNCHAV, KAPPA = 0, 1, 2
 
 
 def build():
@@ -64,6 +67,14 @@ def lib():
         L.or_planck.restype = ctypes.c_double
         L.or_a_th.argtypes = [ctypes.c_double] * 3
         L.or_a_th.restype = ctypes.c_double
+        L.or_hyp2f1.argtypes = [ctypes.c_double] * 4
+        L.or_hyp2f1.restype = ctypes.c_double
+        for fn in (L.or_j_kappa, L.or_a_kappa):
+            fn.argtypes = [ctypes.c_double] * 5
+            fn.restype = ctypes.c_double
+        L.or_j_thsynchav.argtypes = [ctypes.c_double] * 4
+        L.or_j_thsynchav.restype = ctypes.c_double
+        L.or_emission.argtypes = [ctypes.c_int] + [ctypes.c_double] * 5 + [_dp, _dp]
         L.or_jansky_factor.argtypes = [P]
         L.or_jansky_factor.restype = ctypes.c_double
         L.or_units.argtypes = [P, _dp]
@@ -85,13 +96,21 @@ def _d(a):
 
 
 def params(grid, *, width, height, inclination, M_unit, R_high, R_low=1.0, MBH=8.9505e39,
-           cam_size=40.0, stepsize=0.03, max_steps=100000, integrator=RK2):
+           cam_size=40.0, stepsize=0.03, max_steps=100000, integrator=RK2, emission=THSYNCH):
     return OrParams(n1=grid.n1, n2=grid.n2, n3=grid.n3, width=width, height=height,
                     max_steps=max_steps, startx1=grid.startx1, startx2=grid.startx2,
                     startx3=grid.startx3, dx1=grid.dx1, dx2=grid.dx2, dx3=grid.dx3, a=grid.a,
                     hslope=grid.hslope, R0=grid.R0, MBH=MBH, M_unit=M_unit, R_high=R_high,
                     R_low=R_low, cam_size_x=cam_size, cam_size_y=cam_size,
-                    inclination=inclination, stepsize=stepsize, integrator=integrator)
+                    inclination=inclination, stepsize=stepsize, integrator=integrator,
+                    emission=emission)
+
+
+def emission(model, B, mu, Te, nu_p, ne):
+    """(j_nu, alpha_nu) in cgs of one sample for an emission model."""
+    j, a = ctypes.c_double(), ctypes.c_double()
+    lib().or_emission(model, B, mu, Te, nu_p, ne, ctypes.byref(j), ctypes.byref(a))
+    return j.value, a.value
 
 
 def metric_dd(P, X):

@@ -14,6 +14,7 @@
  *   GRmath.c:17-27,135-184 (lower_index, BL_to_KS_u)
  *   newtonraphson.c:11-97 (theta -> X2 inversion; solved deterministically here)
  *   radiative_transfer.c:17-119, j_nu.c:15,20-26,130-153,188-200
+ *   j_nu.c:28-106,157-167 (kappa and angle-averaged thermal emission models)
  *   raptor_harm_model.c:190-320, grmhd.c:17-71 (fluid sampling, units)
  *   utilities.c:17-45 (cutoff_inner, R_GRAV, JANSKY_FACTOR)
  * with the reference's discretisation choices kept on purpose (truncating cell
@@ -72,6 +73,7 @@ typedef struct {
     double inclination;       /* degrees */
     double stepsize;          /* STEPSIZE, model.in:12 */
     int integrator;           /* int_method (parameters.h:186-189): 0 RK2, 1 RK4, 2 Verlet */
+    int emission;             /* 0 THSYNCH (the path), 1 THSYNCHAV, 2 kappa; see or_emission */
 } or_params;
 
 /* per-ray counters: RK2 steps, samples inside the grid gate, samples passing
@@ -575,6 +577,119 @@ double or_a_th(double j, double nu, double Te)
     return j * OR_E2_CH / Bn / OR_E2_C;
 }
 
+/* ---------------------------------------------- other emission models */
+/* Gauss hypergeometric series sum_n (a)_n (b)_n / ((c)_n n!) x^n, 0 <= x <= 1/2. */
+static double or_2f1_series(double a, double b, double c, double x)
+{
+    double term = 1., sum = 1.;
+    for (int n = 0; n < 2000; n++) {
+        term *= (a + n) * (b + n) / ((c + n) * (n + 1.)) * x;
+        sum += term;
+        if (fabs(term) <= 1e-17 * fabs(sum))
+            break;
+    }
+    return sum;
+}
+
+/* 2F1(a, b; c; z) for z <= 0 (the kappa absorptivity's z = -kappa Theta_e),
+ * standing in for gsl_sf_hyperg_2F1 (GSL, version unpinned, absent here) at
+ * j_nu.c:81-91.  -1 <= z <= 0: Pfaff's transformation (Abramowitz & Stegun
+ * 15.3.4) 2F1(a, b; c; z) = (1-z)^-a 2F1(a, c-b; c; z/(z-1)), argument in
+ * [0, 1/2]; z < -1: the continuation in 1/(1-z) (A&S 15.3.8) exactly as
+ * j_nu.c:81-88 writes it, the inner series' argument in (0, 1/2).  b - a must
+ * not be an integer (it is 4/3 for the kappa model). */
+double or_hyp2f1(double a, double b, double c, double z)
+{
+    if (z >= -1.)
+        return pow(1. - z, -a) * or_2f1_series(a, c - b, c, z / (z - 1.));
+    const double x = 1. / (1. - z);
+    return pow(1. - z, -a) * tgamma(c) * tgamma(b - a) / (tgamma(b) * tgamma(c - a)) *
+               or_2f1_series(a, c - b, a - b + 1., x) +
+           pow(1. - z, -b) * tgamma(c) * tgamma(a - b) / (tgamma(a) * tgamma(c - b)) *
+               or_2f1_series(b, c - a, b - a + 1., x);
+}
+
+/* j_nu.c:28-61 emission_coeff_kappa_FIT (Pandya et al. 2016), kappa = 3.5,
+ * width w = Theta_e; mu = cos pitch angle (the reference's `theta`).  cgs. */
+double or_j_kappa(double nu, double Ne, double Te, double B, double mu)
+{
+    const double kappa = 3.5, w = Te;
+    double nuc = OR_ELECTRON_CHARGE * B / (2. * OR_PI * OR_ELECTRON_MASS * OR_C);
+    double sth = sqrt(1 - mu * mu);
+    double factor = (Ne * pow(OR_ELECTRON_CHARGE, 2.) * nuc * sth) / OR_C;
+    double nus = nuc * sth * pow(w * kappa, 2);
+    if (nu == 1.0 || fabs(mu) == 1)
+        return 0;
+    double X = nu / nus;
+    double J_low = pow(X, 1. / 3.) * 4 * OR_PI * tgamma(kappa - 4. / 3.) /
+                   (pow(3, 7. / 3.) * tgamma(kappa - 2.));
+    double J_high = pow(X, -(kappa - 2) / 2.) * pow(3, (kappa - 1) / 2.) * (kappa - 2.) *
+                    (kappa - 1.) / 4. * tgamma(kappa / 4. - 1. / 3.) *
+                    tgamma(kappa / 4. + 4. / 3.);
+    double x = 3 * pow(kappa, -3. / 2.);
+    double J_s = pow((pow(J_low, -x) + pow(J_high, -x)), -1. / x);
+    return J_s * factor;
+}
+
+/* j_nu.c:63-106 absorption_coeff_kappa_FIT (Pandya et al. 2016).  cgs. */
+double or_a_kappa(double nu, double Ne, double Te, double B, double mu)
+{
+    const double kappa = 3.5, w = Te;
+    double nuc = OR_ELECTRON_CHARGE * B / (2. * OR_PI * OR_ELECTRON_MASS * OR_C);
+    double sth = sqrt(1 - mu * mu);
+    double factor = Ne * OR_ELECTRON_CHARGE / (B * sth);
+    if (nu == 1.0 || fabs(mu) == 1.)
+        return 0;
+    double nus = nuc * sth * pow(w * kappa, 2);
+    double X = nu / nus;
+    double a = kappa - 1. / 3., b = kappa + 1., c = kappa + 2. / 3., z = -kappa * w;
+    if (fabs(z) == 1.)
+        return 0;
+    double hyp = or_hyp2f1(a, b, c, z);
+    double A_low = pow(X, -5. / 3.) * pow(3, 1. / 6.) * (10. / 41.) * pow(2 * OR_PI, 2) /
+                   pow(w * kappa, 16. / 3. - kappa) * (kappa - 2.) * (kappa - 1.) * kappa /
+                   (3. * kappa - 1.) * tgamma(5. / 3.) * hyp;
+    double A_high = pow(X, -(3. + kappa) / 2.) * (pow(OR_PI, 5. / 2.) / 3.) *
+                    ((kappa - 2.) * (kappa - 1.) * kappa / pow(w * kappa, 5.)) *
+                    (2 * tgamma(2. + kappa / 2.) / (2. + kappa) - 1.) *
+                    (pow(3. / kappa, 19. / 4.) + 3. / 5.);
+    double x = pow(-7. / 4. + 8. * kappa / 5., -43. / 50.);
+    double A_s = pow((pow(A_low, -x) + pow(A_high, -x)), -1. / x);
+    return factor * A_s;
+}
+
+/* j_nu.c:157-167 emission_coeff_THSYNCHAV (Dexter 2009, angle-averaged), as
+ * written: without the e^2/c that emission_coeff_THSYNCH multiplies by. */
+double or_j_thsynchav(double B, double Te, double nu_p, double ne)
+{
+    double nu_c = 3. * OR_ELECTRON_CHARGE * B * Te * Te / (4. * OR_PI * OR_ELECTRON_MASS * OR_C);
+    double x_M = nu_p / nu_c;
+    double I = 4.0505 / pow(x_M, 1. / 6.) * (1. + 0.4 / pow(x_M, 1. / 4.) + 0.5316 / sqrt(x_M)) *
+               exp(-1.8899 * pow(x_M, 1. / 3.));
+    return nu_p * ne / (2. * sqrt(3.)) * 1. / (Te * Te) * I;
+}
+
+/* The emission model of a sample: emission_coeff (j_nu.c:20-26) calls the
+ * thermal synchrotron fit and Kirchhoff's absorptivity; the reference defines
+ * the other two models but calls neither.  Here they can be chosen:
+ *   1  THSYNCHAV: j = e^2/c * emission_coeff_THSYNCHAV, in cgs like the
+ *      thermal fit, so that absorption_coeff_TH (Kirchhoff) applies as is;
+ *   2  kappa: emissivity and absorptivity fits of j_nu.c:28-106. */
+void or_emission(int model, double B, double mu, double Te, double nu_p, double ne, double *j,
+                 double *an)
+{
+    if (model == 2) {
+        *j = or_j_kappa(nu_p, ne, Te, B, mu);
+        *an = or_a_kappa(nu_p, ne, Te, B, mu);
+    } else if (model == 1) {
+        *j = OR_E2_C * or_j_thsynchav(B, Te, nu_p, ne);
+        *an = or_a_th(*j, nu_p, Te);
+    } else {
+        *j = or_j_thsynch(B, mu, Te, nu_p, ne);
+        *an = or_a_th(*j, nu_p, Te);
+    }
+}
+
 /* d ln j / d ln X of the THSYNCH fit at fixed n_e, nu_s:
  * ln f = 2 ln(X^1/2 + 2^(11/12) X^1/6), minus X^1/3. */
 static double or_dlnj_dlnX(double X)
@@ -686,8 +801,8 @@ void or_trace_ray(const or_params *P, const double *prims, int x, int yy, int nf
                     nu_p *= -mc2_h;
                     if (nu_p <= 0.0)
                         nu_p = 1.0;
-                    double j = or_j_thsynch(F.B, mu, F.thetae, nu_p, F.ne);
-                    double an = or_a_th(j, nu_p, F.thetae);
+                    double j, an;
+                    or_emission(P->emission, F.B, mu, F.thetae, nu_p, F.ne, &j, &an);
                     double dlf = dlabs * D.Rg / freqs[f];
                     double dtau, dIk;
                     const double tau0 = tau[f];
@@ -743,6 +858,8 @@ int or_render(const or_params *P, const double *prims, int nf, const double *fre
               double *gate, int nthreads)
 {
     if (nf < 1 || nf > 64 || npix < 0)
+        return -1;
+    if (dimage && P->emission != 0)   /* the tangents are the thermal fit's */
         return -1;
 #ifdef _OPENMP
     if (nthreads > 0)

@@ -267,7 +267,7 @@ constexpr bool three_waves()
     return INT == kRK2 && NF == 1 && !(GRAD && DIM3);
 }
 
-template <int NF, bool GRAD, bool DIM3, int INT, bool RESUME>
+template <int NF, bool GRAD, bool DIM3, int INT, bool RESUME, int EMIS>
 __device__ __forceinline__ void trace_rays(const Params &P, const TraceArgs &A)
 {
     Ray<NF, GRAD> R;
@@ -280,7 +280,7 @@ __device__ __forceinline__ void trace_rays(const Params &P, const TraceArgs &A)
         if (active) {
             bool cont = ray_alive<NF, GRAD>(P, R);
             if (cont)
-                cont = ray_step<NF, GRAD, DIM3, INT>(P, A.prims, R);
+                cont = ray_step<NF, GRAD, DIM3, INT, EMIS>(P, A.prims, R);
             if (!cont) {
                 write_ray<NF, GRAD, DIM3>(P, A, ray, R);
                 active = false;
@@ -313,16 +313,16 @@ __device__ __forceinline__ void trace_rays(const Params &P, const TraceArgs &A)
     }
 }
 
-template <int NF, bool GRAD, bool DIM3, int INT, bool RESUME>
+template <int NF, bool GRAD, bool DIM3, int INT, bool RESUME, int EMIS>
 __global__ __launch_bounds__(64) void trace_kernel(const Params P, const TraceArgs A)
 {
-    trace_rays<NF, GRAD, DIM3, INT, RESUME>(P, A);
+    trace_rays<NF, GRAD, DIM3, INT, RESUME, EMIS>(P, A);
 }
 
-template <int NF, bool GRAD, bool DIM3, int INT, bool RESUME>
+template <int NF, bool GRAD, bool DIM3, int INT, bool RESUME, int EMIS>
 __global__ __launch_bounds__(64, 3) void trace_kernel_3w(const Params P, const TraceArgs A)
 {
-    trace_rays<NF, GRAD, DIM3, INT, RESUME>(P, A);
+    trace_rays<NF, GRAD, DIM3, INT, RESUME, EMIS>(P, A);
 }
 
 __global__ void reset_queue_kernel(unsigned int *q) { *q = 0u; }
@@ -391,6 +391,28 @@ __global__ void probe_camera(const Params P, const double *in, int n, double *ou
         return;
     int cam = (int)in[3 * t + 2];
     init_photon(P.cam[cam], in[3 * t], in[3 * t + 1], out + 8 * t);
+}
+
+__global__ void probe_emission_model(const KappaConst K, const double *in, int n, double *out)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    const double *q = in + 6 * t;
+    const int model = (int)q[5];
+    const float B = (float)q[0], mu = (float)q[1], Te = (float)q[2], nu = (float)q[3],
+                ne = (float)q[4];
+    if (model == kThSynch) {
+        EmisF e = emission_f32(B, mu, Te, nu, ne);
+        double jx = (double)e.jh * exp(-(double)e.cb);
+        out[2 * t] = kE2C * jx;
+        out[2 * t + 1] = e.jh != 0.f ? jx * (double)e.kap : (e.poison ? NAN : 0.);
+        return;
+    }
+    const EmisX e = model == kKappa ? emission_kappa(K, B, mu, Te, nu, ne)
+                                    : emission_thsynchav(B, Te, nu, ne);
+    out[2 * t] = kE2C * (double)e.jx;
+    out[2 * t + 1] = (model == kKappa || e.jx != 0.f) ? (double)e.alpha : (e.poison ? NAN : 0.);
 }
 
 __global__ void probe_emission(const double *in, int n, double *out)
@@ -556,13 +578,13 @@ static int make_params(rt_ctx *c, const double *incl, int ncam, const double *fr
     return RT_OK;
 }
 
-template <int NF, bool GRAD, bool DIM3, int INT, bool RESUME>
+template <int NF, bool GRAD, bool DIM3, int INT, bool RESUME, int EMIS = kThSynch>
 static hipError_t launch_trace(rt_ctx *c, const Params &P, const TraceArgs &A)
 {
-    void (*kern)(const Params, const TraceArgs) = trace_kernel<NF, GRAD, DIM3, INT, RESUME>;
-    if constexpr (three_waves<NF, GRAD, DIM3, INT>()) {
+    void (*kern)(const Params, const TraceArgs) = trace_kernel<NF, GRAD, DIM3, INT, RESUME, EMIS>;
+    if constexpr (EMIS == kThSynch && three_waves<NF, GRAD, DIM3, INT>()) {
         if (A.total >= kThreeWaveRays)
-            kern = trace_kernel_3w<NF, GRAD, DIM3, INT, RESUME>;
+            kern = trace_kernel_3w<NF, GRAD, DIM3, INT, RESUME, EMIS>;
     }
     int block = 64;
     int per_cu = c->cfg.blocks_per_cu;
@@ -580,9 +602,22 @@ static hipError_t launch_trace(rt_ctx *c, const Params &P, const TraceArgs &A)
     return hipGetLastError();
 }
 
+// The thermal fit is built for 1, 4, 8 and 16 frequencies, image and
+// gradient; the other emission models (image only) for 1 and 16.
 template <bool GRAD, bool DIM3, int INT, bool RESUME>
 static hipError_t dispatch_nf(rt_ctx *c, const Params &P, const TraceArgs &A)
 {
+    if (P.emission != kThSynch) {
+        if constexpr (GRAD) {
+            return hipErrorInvalidValue;   // refused before the launch
+        } else {
+            if (P.emission == kKappa)
+                return P.nf == 1 ? launch_trace<1, false, DIM3, INT, RESUME, kKappa>(c, P, A)
+                                 : launch_trace<16, false, DIM3, INT, RESUME, kKappa>(c, P, A);
+            return P.nf == 1 ? launch_trace<1, false, DIM3, INT, RESUME, kThSynchAv>(c, P, A)
+                             : launch_trace<16, false, DIM3, INT, RESUME, kThSynchAv>(c, P, A);
+        }
+    }
     if (P.nf == 1)
         return launch_trace<1, GRAD, DIM3, INT, RESUME>(c, P, A);
     if (P.nf <= 4)
@@ -783,6 +818,9 @@ static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *fr
         return rc;
     if (!d_image || (grad && !d_dimage))
         return fail(c, RT_E_INVALID, "output buffers must not be NULL");
+    if (grad && c->params.emission != RT_EMIS_THSYNCH)
+        return fail(c, RT_E_INVALID, "gradients are available for the thermal synchrotron "
+                                     "emission model (RT_EMIS_THSYNCH) only");
     if ((rc = select_device(c)))
         return rc;
     A.prims = c->d_prims;
@@ -965,6 +1003,9 @@ int rt_set_params(rt_ctx *c, const rt_params *p)
     if (p->integrator < RT_INT_RK2 || p->integrator > RT_INT_VERLET)
         return fail(c, RT_E_INVALID, "integrator must be RT_INT_RK2, RT_INT_RK4 or RT_INT_VERLET, "
                                      "got %d", p->integrator);
+    if (p->emission < RT_EMIS_THSYNCH || p->emission > RT_EMIS_KAPPA)
+        return fail(c, RT_E_INVALID, "emission must be RT_EMIS_THSYNCH, RT_EMIS_THSYNCHAV or "
+                                     "RT_EMIS_KAPPA, got %d", p->emission);
     c->params = *p;
     c->have_params = true;
     return RT_OK;
@@ -1303,6 +1344,7 @@ int rt_probe(rt_ctx *c, int kind, const double *h_in, int n, double *h_out)
     case RT_PROBE_RK2: nin = 8; nout = 9; break;
     case RT_PROBE_CAMERA: nin = 3; nout = 8; break;
     case RT_PROBE_EMISSION: nin = 5; nout = 2; break;
+    case RT_PROBE_EMISSION_MODEL: nin = 6; nout = 2; break;
     default: return fail(c, RT_E_INVALID, "unknown probe kind %d", kind);
     }
     if (n == 0)
@@ -1310,7 +1352,13 @@ int rt_probe(rt_ctx *c, int kind, const double *h_in, int n, double *h_out)
     Params P;
     memset(&P, 0, sizeof P);
     std::vector<double> in(h_in, h_in + (size_t)nin * n);
-    if (kind != RT_PROBE_EMISSION) {
+    if (kind == RT_PROBE_EMISSION_MODEL) {
+        for (int i = 0; i < n; i++) {
+            const double m = in[6 * i + 5];
+            if (m != RT_EMIS_THSYNCH && m != RT_EMIS_THSYNCHAV && m != RT_EMIS_KAPPA)
+                return fail(c, RT_E_INVALID, "unknown emission model %g", m);
+        }
+    } else if (kind != RT_PROBE_EMISSION) {
         if (!c->have_grid || !c->have_params)
             return fail(c, RT_E_STATE, "rt_set_params and rt_load_prims must be called first");
         std::vector<double> incl;
@@ -1358,6 +1406,10 @@ int rt_probe(rt_ctx *c, int kind, const double *h_in, int n, double *h_out)
             break;
         case RT_PROBE_CAMERA:
             hipLaunchKernelGGL(probe_camera, grid, block, 0, c->stream, P, d_in, n, d_out);
+            break;
+        case RT_PROBE_EMISSION_MODEL:
+            hipLaunchKernelGGL(probe_emission_model, grid, block, 0, c->stream, kappa_const(),
+                               d_in, n, d_out);
             break;
         default:
             hipLaunchKernelGGL(probe_emission, grid, block, 0, c->stream, d_in, n, d_out);

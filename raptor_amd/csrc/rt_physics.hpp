@@ -63,6 +63,24 @@ struct Camera {
 };
 
 // Everything a trace launch reads; passed by value as the kernel argument.
+// Emission models (rt_params.emission).  emission_coeff (j_nu.c:20-26) calls
+// the thermal synchrotron fit; the reference also defines an angle-averaged
+// thermal fit and the kappa-distribution fits, which nothing on its path
+// calls, and which can be chosen here.
+constexpr int kThSynch = 0, kThSynchAv = 1, kKappa = 2;
+
+// Constants of the kappa = 3.5 fits (j_nu.c:28-106), computed on the host in
+// FP64 (kappa_const) and used in FP32 in log2 form, so that the power laws
+// stay in range down to the Theta_e floor of 1e-20.
+struct KappaConst {
+    float lJlow, lJhigh;   // log2 of J_low / X^(1/3) and J_high / X^(-(kappa-2)/2)
+    float xj;              // J_s = (J_low^-xj + J_high^-xj)^(-1/xj)
+    float lAlow, lAhigh;   // log2 of the constant factors of A_low and A_high
+    float xa;              // A_s exponent
+    float ha, hb, hc;      // hypergeometric parameters kappa - 1/3, kappa + 1, kappa + 2/3
+    float g1, g2;          // Gamma-function ratios of the 1/(1-z) continuation
+};
+
 struct Params {
     // grid (raptor_harm_model.c:120-135)
     int n1, n2, n3;
@@ -85,6 +103,8 @@ struct Params {
     // FP32 copies for the transfer
     float fnu[kMaxFreq];
     float fNe_unit, fB_unit, fiB_unit, fR_high, fR_low, fa, fpi_h;   // fpi_h = (1 - h) pi
+    int emission;                // kThSynch, kThSynchAv or kKappa
+    KappaConst kc;
     int ncam;
     Camera cam[kMaxCam];
 };
@@ -643,8 +663,9 @@ __host__ __device__ inline bool fluid_sample(const Params &P, const float4 *__re
     float ku = kd[0] * u[0] + kd[1] * u[1] + kd[2] * u[2] + kd[3] * u[3];
     float mu = (kd[0] * b[0] + kd[1] * b[1] + kd[2] * b[2] + kd[3] * b[3]) *
                frcp(fabsf(ku) * (F.B * P.fiB_unit));
+    // radiative_transfer.c:38-42 clamps with mu / |mu|, which is exactly +-1
     if (fabsf(mu) > 1.f)
-        mu /= fabsf(mu);
+        mu = copysignf(1.f, mu);
     F.mu = mu;
     F.gdop = -ku;
     return true;
@@ -705,6 +726,110 @@ __host__ __device__ inline float dlnj_dlnX(float X, float cb)
     return 2.f * (0.5f * sx + c1112 / 6.f * t) * frcp(sx + c1112 * t) - cb * (1.f / 3.f);
 }
 
+// Emissivity and absorptivity of the other emission models: j_nu = (e^2/c) jx
+// and alpha_nu, both cgs; `poison` as for the thermal fit.
+struct EmisX {
+    float jx, alpha;
+    bool poison;
+};
+
+// log2(2^p + 2^q) without overflow
+__host__ __device__ inline float log2_add(float p, float q)
+{
+    return fmaxf(p, q) + flog2(1.f + fexp2(-fabsf(p - q)));
+}
+
+// Gauss series sum_n (a)_n (b)_n / ((c)_n n!) x^n for 0 <= x <= 1/2, FP32.
+__host__ __device__ inline float hyp_series(float a, float b, float c, float x)
+{
+    float term = 1.f, sum = 1.f;
+    for (int n = 0; n < 96; n++) {
+        const float fn = (float)n;
+        term *= (a + fn) * (b + fn) * frcp((c + fn) * (fn + 1.f)) * x;
+        sum += term;
+        if (fabsf(term) <= 3e-8f * fabsf(sum))
+            break;
+    }
+    return sum;
+}
+
+// 2F1(a, b; c; z) of the kappa absorptivity, z = -kappa Theta_e < 0
+// (j_nu.c:76-91): Pfaff's transformation for -1 <= z < 0, the 1/(1-z)
+// continuation the reference writes out for z < -1.
+__host__ __device__ inline float kappa_hyp(const KappaConst &K, float z)
+{
+    const float l1z = flog2(1.f - z);
+    if (z >= -1.f)
+        return fexp2(-K.ha * l1z) * hyp_series(K.ha, K.hc - K.hb, K.hc, z * frcp(z - 1.f));
+    // the first term's 2F1(a, c-b; a-b+1; x) has c - b = a - b + 1 (-1/3 for
+    // any kappa), so it is (1-x)^-a and the term is g1 (-z)^-a
+    const float x = frcp(1.f - z);
+    return fexp2(-K.ha * flog2(-z)) * K.g1 +
+           fexp2(-K.hb * l1z) * K.g2 * hyp_series(K.hb, K.hc - K.ha, K.hb - K.ha + 1.f, x);
+}
+
+// emission_coeff_kappa_FIT and absorption_coeff_kappa_FIT (j_nu.c:28-106,
+// Pandya et al. 2016), kappa = 3.5, width w = Theta_e, mu = cos(pitch).
+// j = (e^2/c) n_e nu_c sin J_s, alpha = n_e e / (B sin) A_s, with the
+// power laws in X = nu / nu_s, nu_s = nu_c sin (w kappa)^2, taken in log2.
+__host__ __device__ inline EmisX emission_kappa(const KappaConst &K, float B, float mu, float Te,
+                                                float nu_p, float ne)
+{
+    EmisX e{0.f, 0.f, false};
+    if (nu_p == 1.f || fabsf(mu) == 1.f)
+        return e;
+    const float kap = 3.5f;
+    const float sth = fsqrt(1.f - mu * mu);
+    const float nuc = (float)(kElectronCharge / (2. * kPi * kElectronMass * kC)) * B;
+    const float wk = Te * kap;
+    const float lwk = flog2(wk);
+    const float lX = flog2(nu_p) - flog2(nuc * sth) - 2.f * lwk;
+    const float lJ = -log2_add(-K.xj * (lX * (1.f / 3.f) + K.lJlow),
+                               -K.xj * (-0.5f * (kap - 2.f) * lX + K.lJhigh)) *
+                     frcp(K.xj);
+    e.jx = fexp2(lJ) * (ne * nuc * sth);
+    const float z = -wk;
+    if (fabsf(z) == 1.f)
+        return e;   // j_nu.c:79: the absorptivity is 0 there
+    const float lhyp = flog2(kappa_hyp(K, z));
+    const float lAl = -(5.f / 3.f) * lX + K.lAlow - (16.f / 3.f - kap) * lwk + lhyp;
+    const float lAh = -0.5f * (3.f + kap) * lX + K.lAhigh - 5.f * lwk;
+    const float lA = -log2_add(-K.xa * lAl, -K.xa * lAh) * frcp(K.xa);
+    e.alpha = fexp2(lA + flog2(ne * (float)kElectronCharge * frcp(B * sth)));
+    return e;
+}
+
+// emission_coeff_THSYNCHAV (j_nu.c:157-167, Dexter 2009) times e^2/c, with
+// Kirchhoff's absorptivity (j_nu.c:194-200) as for the thermal fit.
+__host__ __device__ inline EmisX emission_thsynchav(float B, float Te, float nu_p, float ne)
+{
+    EmisX e{0.f, 0.f, false};
+    const float lnuc = flog2((float)(3. * kElectronCharge / (4. * kPi * kElectronMass * kC)) * B);
+    const float lTe = flog2(Te);
+    const float lx = flog2(nu_p) - lnuc - 2.f * lTe;   // log2 x_M
+    const float I = 4.0505f * fexp2(-lx * (1.f / 6.f)) *
+                    (1.f + 0.4f * fexp2(-0.25f * lx) + 0.5316f * fexp2(-0.5f * lx)) *
+                    fexp(-1.8899f * fexp2(lx * (1.f / 3.f)));
+    // n_e nu_p I / (2 sqrt 3 Theta_e^2), in log2 so that Theta_e^2 cannot
+    // underflow where I has
+    e.jx = fexp2(flog2(I) + flog2(nu_p * ne) - 2.f * lTe) * (float)(0.5 / 1.7320508075688772);
+    const float inu = frcp(nu_p);
+    const float x = (float)(kPlanck / (kElectronMass * kC * kC)) * nu_p * frcp(Te);
+    const float cn = (float)kC * inu;
+    if (e.jx != 0.f) {
+        // below 2^-53 the reference's exp(x) - 1 (j_nu.c:190) is exactly 0, its
+        // B_nu infinite and its absorptivity 0 (nu_p = 1 Hz, the floor of
+        // radiative_transfer.c:24-26, where this fit has no gate)
+        e.alpha = x < 1.1102230246251565e-16f
+                      ? 0.f
+                      : e.jx * ((float)(0.5 * kE2CH) * expm1f(x) * (cn * cn) * inu);
+    } else {
+        double T = (double)Te * kElectronMass * kC * kC / kBoltzmann;
+        e.poison = kPlanck * (double)nu_p / (kBoltzmann * T) > 709.782712893384;
+    }
+    return e;
+}
+
 // Per-ray accumulators for NF frequencies: optical depth and I_nu nu^3
 // (FP32 with Kahan compensation), and the tangents (FP64: they difference
 // nearly equal terms, d ln j - d tau).
@@ -740,7 +865,7 @@ __host__ __device__ inline void kahan_add(float &s, float &c, float x)
 // dl = |dlambda| R_g:
 //   dtau = nu_p alpha_nu dl / nu = g alpha_nu dl
 //   dI nu^3 = (j_nu / nu_p^2) exp(-tau) (dl / nu) nu^3 = j_nu exp(-tau) dl / g^2
-template <int NF, bool GRAD>
+template <int NF, bool GRAD, int EMIS = kThSynch>
 __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F, double dlabs,
                                                 Accum<NF, GRAD> &acc)
 {
@@ -753,6 +878,22 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
         if (nu_p <= 0.f) {
             nu_p = 1.f;
             g = frcp(P.fnu[f]);
+        }
+        if constexpr (EMIS != kThSynch) {
+            static_assert(!GRAD, "the tangents are those of the thermal fit");
+            if (f >= P.nf)
+                break;
+            const EmisX e = EMIS == kKappa
+                                ? emission_kappa(P.kc, F.B, F.mu, F.thetae, nu_p, F.ne)
+                                : emission_thsynchav(F.B, F.thetae, nu_p, F.ne);
+            const float dtau = (EMIS == kKappa || e.jx != 0.f) ? g * dl * e.alpha
+                                                               : (e.poison ? NAN : 0.f);
+            kahan_add(acc.tau[f], acc.ctau[f], dtau);
+            const float tau = acc.tau[f];
+            const float dI = (float)kE2C * e.jx * fexp(-tau) * (dl * frcp(g * g));
+            if (tau < lnth && !(tau < 0.f))
+                kahan_add(acc.I[f], acc.cI[f], dI);
+            continue;
         }
         const EmisF e = emission_f32(F.B, F.mu, F.thetae, nu_p, F.ne);
         const float jx = e.jh * fexp(-e.cb);
@@ -859,7 +1000,7 @@ __host__ __device__ inline bool in_grid(const Params &P, const double y[8])
 
 // The rest of the iteration (integrator.c:277-284): the sample of the new
 // point, when it is inside the grid, with the step's |dlambda|.
-template <int NF, bool GRAD, bool DIM3>
+template <int NF, bool GRAD, bool DIM3, int EMIS = kThSynch>
 __host__ __device__ inline void ray_sample(const Params &P, const float4 *__restrict__ prims,
                                            Ray<NF, GRAD> &R, double dl)
 {
@@ -867,7 +1008,7 @@ __host__ __device__ inline void ray_sample(const Params &P, const float4 *__rest
         Fluid F;
         if (fluid_sample<DIM3>(P, prims, R.pt, R.y, F)) {
             R.samples++;
-            transfer_sample<NF, GRAD>(P, F, fabs(dl), R.acc);
+            transfer_sample<NF, GRAD, EMIS>(P, F, fabs(dl), R.acc);
         }
     }
     R.steps++;
@@ -875,14 +1016,14 @@ __host__ __device__ inline void ray_sample(const Params &P, const float4 *__rest
 
 // One iteration of integrate_geodesic's loop (integrator.c:243-284).
 // Returns false when the ray escaped through the outer grid edge.
-template <int NF, bool GRAD, bool DIM3, int INT = kRK2>
+template <int NF, bool GRAD, bool DIM3, int INT = kRK2, int EMIS = kThSynch>
 __host__ __device__ inline bool ray_step(const Params &P, const float4 *__restrict__ prims,
                                          Ray<NF, GRAD> &R)
 {
     double dl;
     if (!ray_advance<INT>(P, R, dl))
         return false;
-    ray_sample<NF, GRAD, DIM3>(P, prims, R, dl);
+    ray_sample<NF, GRAD, DIM3, EMIS>(P, prims, R, dl);
     return true;
 }
 
@@ -981,6 +1122,34 @@ inline Camera make_camera(double incl_deg, double a, double h)
 
 // Host-side set-up of a launch from the C-ABI structs (grmhd.c:17-33 units,
 // utilities.c:17-21 cutoff).  Returns NULL or an error message.
+// The kappa = 3.5 fits' constant factors (j_nu.c:35-100), FP64 on the host.
+inline KappaConst kappa_const()
+{
+    const double k = 3.5;
+    KappaConst K;
+    const double Jl = 4. * kPi * std::tgamma(k - 4. / 3.) / (std::pow(3., 7. / 3.) * std::tgamma(k - 2.));
+    const double Jh = std::pow(3., (k - 1.) / 2.) * (k - 2.) * (k - 1.) / 4. *
+                      std::tgamma(k / 4. - 1. / 3.) * std::tgamma(k / 4. + 4. / 3.);
+    K.lJlow = (float)std::log2(Jl);
+    K.lJhigh = (float)std::log2(Jh);
+    K.xj = (float)(3. * std::pow(k, -1.5));
+    const double Al = std::pow(3., 1. / 6.) * (10. / 41.) * std::pow(2. * kPi, 2) * (k - 2.) *
+                      (k - 1.) * k / (3. * k - 1.) * std::tgamma(5. / 3.);
+    const double Ah = (std::pow(kPi, 2.5) / 3.) * (k - 2.) * (k - 1.) * k *
+                      (2. * std::tgamma(2. + k / 2.) / (2. + k) - 1.) *
+                      (std::pow(3. / k, 19. / 4.) + 3. / 5.);
+    K.lAlow = (float)std::log2(Al);
+    K.lAhigh = (float)std::log2(Ah);
+    K.xa = (float)std::pow(-7. / 4. + 8. * k / 5., -43. / 50.);
+    const double a = k - 1. / 3., b = k + 1., c = k + 2. / 3.;
+    K.ha = (float)a;
+    K.hb = (float)b;
+    K.hc = (float)c;
+    K.g1 = (float)(std::tgamma(c) * std::tgamma(b - a) / (std::tgamma(b) * std::tgamma(c - a)));
+    K.g2 = (float)(std::tgamma(c) * std::tgamma(a - b) / (std::tgamma(a) * std::tgamma(c - b)));
+    return K;
+}
+
 template <class RtParams, class RtGrid>
 inline const char *fill_params(const RtParams &p, const RtGrid &g, const double *incl, int ncam,
                                const double *freqs, int nf, Params &P)
@@ -1025,6 +1194,8 @@ inline const char *fill_params(const RtParams &p, const RtGrid &g, const double 
     P.fR_low = (float)p.R_low;
     P.fa = (float)g.a;
     P.fpi_h = (float)((1. - g.hslope) * kPi);
+    P.emission = p.emission;
+    P.kc = kappa_const();
     P.nf = nf;
     for (int f = 0; f < kMaxFreq; f++) {
         double nu = freqs[std::min(f, nf - 1)];

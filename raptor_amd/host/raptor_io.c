@@ -78,7 +78,7 @@ int rio_parse_args(int argc, char **argv, rio_model *m, char *err, size_t errlen
         return seterr(err, errlen,
                       "usage: %s model.in dumpfile M_UNIT INCLINATION R_HIGH R_LOW TIME_INIT "
                       "[--freqs f1,f2,..] [--nfreq N] [--incl i1,i2,..] [--device D] "
-                      "[--integrator rk2|rk4|verlet]",
+                      "[--integrator rk2|rk4|verlet] [--emission thsynch|thsynchav|kappa]",
                       argc ? argv[0] : "raptor");
     if (strlen(argv[2]) >= sizeof m->GRMHD_FILE)
         return seterr(err, errlen, "dump file name too long");
@@ -95,6 +95,7 @@ int rio_parse_args(int argc, char **argv, rio_model *m, char *err, size_t errlen
     m->incl[0] = m->INCLINATION;
     m->device = 0;
     m->integrator = RT_INT_RK2;
+    m->emission = RT_EMIS_THSYNCH;
     for (int i = 8; i < argc; i++) {
         if (i + 1 >= argc)
             return seterr(err, errlen, "flag %s needs a value", argv[i]);
@@ -127,6 +128,15 @@ int rio_parse_args(int argc, char **argv, rio_model *m, char *err, size_t errlen
                 m->integrator = RT_INT_VERLET;
             else
                 return seterr(err, errlen, "--integrator must be rk2, rk4 or verlet");
+        } else if (!strcmp(argv[i - 1], "--emission")) {
+            if (!strcmp(v, "thsynch"))
+                m->emission = RT_EMIS_THSYNCH;
+            else if (!strcmp(v, "thsynchav"))
+                m->emission = RT_EMIS_THSYNCHAV;
+            else if (!strcmp(v, "kappa"))
+                m->emission = RT_EMIS_KAPPA;
+            else
+                return seterr(err, errlen, "--emission must be thsynch, thsynchav or kappa");
         } else {
             return seterr(err, errlen, "unknown flag %s", argv[i - 1]);
         }

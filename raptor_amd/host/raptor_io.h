@@ -28,6 +28,7 @@ typedef struct {
     double incl[RIO_MAX_CAM];
     int device;
     int integrator;               /* RT_INT_* (int_method, parameters.h:186-189) */
+    int emission;                 /* RT_EMIS_* (j_nu.c emission models) */
 } rio_model;
 
 /* A primitive set in the reference's index order p[v][i][j][k]. */
@@ -47,6 +48,8 @@ int rio_read_model_in(const char *path, rio_model *m, char *err, size_t errlen);
  *   --incl i1,i2,...    several inclinations in one launch
  *   --device D
  *   --integrator rk2|rk4|verlet   the stepper (int_method); rk2 by default
+ *   --emission thsynch|thsynchav|kappa   the emission model; thsynch (the
+ *                       one emission_coeff calls) by default
  * argv: prog model.in dump M_UNIT INCLINATION R_HIGH R_LOW TIME_INIT [flags] */
 int rio_parse_args(int argc, char **argv, rio_model *m, char *err, size_t errlen);
 

@@ -30,6 +30,7 @@ static void fill_params(const rio_model *m, rt_params *p)
     p->stepsize = m->STEPSIZE;
     p->max_steps = 100000; /* parameters.h:182 */
     p->integrator = m->integrator;
+    p->emission = m->emission;
 }
 
 int init_model(rt_ctx *ctx, const rio_model *m, const rio_dump *d)

@@ -21,6 +21,7 @@ RT_OK = 0
 RT_MAX_FREQ = 16
 RT_MAX_CAMERAS = 64
 PROBE_METRIC, PROBE_ACCEL, PROBE_RK2, PROBE_CAMERA, PROBE_EMISSION = 1, 2, 3, 4, 5
+PROBE_EMISSION_MODEL = 6
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
@@ -40,7 +41,7 @@ class RtParams(ctypes.Structure):
                 ("width", ctypes.c_int), ("height", ctypes.c_int),
                 ("cam_size_x", ctypes.c_double), ("cam_size_y", ctypes.c_double),
                 ("stepsize", ctypes.c_double), ("max_steps", ctypes.c_int),
-                ("integrator", ctypes.c_int)]
+                ("integrator", ctypes.c_int), ("emission", ctypes.c_int)]
 
 
 class RtLaunchConfig(ctypes.Structure):
@@ -51,6 +52,9 @@ class RtLaunchConfig(ctypes.Structure):
 
 # rt_params.integrator (include/raptor_rt.h): int_method of parameters.h:186-189
 INT_RK2, INT_RK4, INT_VERLET = 0, 1, 2
+# rt_params.emission: the thermal fit of the path, the angle-averaged thermal
+# fit and the kappa fits of j_nu.c
+EMIS_THSYNCH, EMIS_THSYNCHAV, EMIS_KAPPA = 0, 1, 2
 
 # claim orders of rt_launch_config.order (include/raptor_rt.h)
 ORDER_CENTRE = 1
@@ -171,11 +175,12 @@ class Tracer:
     # -- set-up ---------------------------------------------------------------
     def set_params(self, *, width, height, M_unit, R_high, R_low=1.0, MBH=8.9505e39,
                    cam_size=40.0, cam_size_y=None, stepsize=0.03, max_steps=100000,
-                   integrator=INT_RK2):
+                   integrator=INT_RK2, emission=EMIS_THSYNCH):
         p = RtParams(MBH=MBH, M_unit=M_unit, R_high=R_high, R_low=R_low, width=width,
                      height=height, cam_size_x=cam_size,
                      cam_size_y=cam_size if cam_size_y is None else cam_size_y,
-                     stepsize=stepsize, max_steps=max_steps, integrator=integrator)
+                     stepsize=stepsize, max_steps=max_steps, integrator=integrator,
+                     emission=emission)
         self._check(self.lib.rt_set_params(self.ctx, ctypes.byref(p)))
         self.width, self.height = width, height
         self.params = p
@@ -269,9 +274,9 @@ class Tracer:
 
     def probe(self, kind, inp):
         nin = {PROBE_METRIC: 4, PROBE_ACCEL: 8, PROBE_RK2: 8, PROBE_CAMERA: 3,
-               PROBE_EMISSION: 5}[kind]
+               PROBE_EMISSION: 5, PROBE_EMISSION_MODEL: 6}[kind]
         nout = {PROBE_METRIC: 32, PROBE_ACCEL: 4, PROBE_RK2: 9, PROBE_CAMERA: 8,
-                PROBE_EMISSION: 2}[kind]
+                PROBE_EMISSION: 2, PROBE_EMISSION_MODEL: 2}[kind]
         inp = np.ascontiguousarray(np.asarray(inp, float).reshape(-1, nin))
         out = np.empty((inp.shape[0], nout))
         self._check(self.lib.rt_probe(self.ctx, kind, _d(inp), inp.shape[0], _d(out)))

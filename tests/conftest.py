@@ -32,6 +32,14 @@ def dump128():
 
 
 @pytest.fixture(scope="session")
+def dump3d_small():
+    from raptor_amd import torus
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        return torus.fishbone_moncrief(48, 32, 16)
+
+
+@pytest.fixture(scope="session")
 def dump256():
     from raptor_amd import torus
     with warnings.catch_warnings():

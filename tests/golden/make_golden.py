@@ -23,7 +23,26 @@ blocks that do not come from the oracle (or from RAPTOR) itself.
        F(x) = x int_x^inf K_{5/3}(t) dt,  nu_c = e B / (2 pi m_e c),
    stored as j_nu / (n_e e^2 nu_c / c), with the exact K2.
 
-Run:  python tests/golden/make_golden.py [kerr] [thsynch]   (about two minutes)
+3. kappa.json -- emissivity and absorptivity of the relativistic kappa
+   distribution n(gamma) = N gamma sqrt(gamma^2-1) (1 + (gamma-1)/(kappa w))^-(kappa+1)
+   (normalised to n_e numerically), kappa = 3.5, from the single-electron
+   spectrum P_nu = 2 pi sqrt(3) e^2 nu_c sin(theta)/c F(x), x = 2 nu / (3
+   gamma^2 nu_c sin(theta)) (Rybicki & Lightman 1979, eq. 6.18):
+       j_nu     = 1/(4 pi) int dgamma n P_nu,
+       alpha_nu = -c^2 / (8 pi nu^2 m_e c^2) int dgamma P_nu gamma^2 d/dgamma(n / gamma^2)
+   (R&L eq. 6.50), stored as J = j_nu / (n_e e^2 nu_c sin(theta) / c) and
+   A = alpha_nu B sin(theta) / (n_e e), the normalisations of the fits of
+   Pandya et al. (2016).
+
+4. thsynchav.json -- the thermal emissivity of item 2 averaged over an
+   isotropic distribution of pitch angles (Gauss-Legendre in cos theta),
+   stored as j_nu / (n_e e^2 nu_c / c) against x_M = 2 nu / (3 nu_c Theta^2).
+
+Items 3-4 tabulate F on a logarithmic grid (cubic interpolation in log-log,
+relative error below 1e-7) to keep the nested integrals fast.
+
+Run:  python tests/golden/make_golden.py [kerr] [thsynch] [kappa] [thsynchav]
+(about two minutes each)
 """
 import json
 import os
@@ -137,10 +156,110 @@ def thsynch_golden():
         json.dump({"rows": rows}, fh, indent=1)
 
 
+def _fsync_table():
+    import numpy as np
+    from scipy import integrate, interpolate, special
+
+    xs = np.logspace(-8, np.log10(60.0), 700)
+    # int_x^inf K_5/3(t) dt in ln t (K_5/3 ~ t^-5/3 near 0; K_5/3(160) < 1e-70)
+    F = np.array([x * integrate.quad(lambda u: special.kv(5.0 / 3.0, np.exp(u)) * np.exp(u),
+                                     np.log(x), np.log(160.0), limit=400, epsabs=0,
+                                     epsrel=1e-12)[0] for x in xs])
+    spl = interpolate.CubicSpline(np.log(xs), np.log(F))
+
+    def Fsync(x):
+        x = np.asarray(x, float)
+        out = np.zeros_like(x)
+        m = (x >= xs[0]) & (x <= xs[-1])
+        out[m] = np.exp(spl(np.log(x[m])))
+        lo = x < xs[0]   # F -> 2^(2/3) pi / (sqrt(3) Gamma(1/3)) x^(1/3)
+        out[lo] = 2 ** (2 / 3) * np.pi / (np.sqrt(3) * special.gamma(1 / 3)) * np.cbrt(x[lo])
+        return out
+    return Fsync
+
+
+def kappa_golden():
+    import numpy as np
+    from scipy import integrate
+
+    Fsync = _fsync_table()
+    kappa = 3.5
+    rows = []
+    for w in (0.2, 1.0, 10.0, 50.0):
+        shape = lambda g: g * np.sqrt(g * g - 1) * (1 + (g - 1) / (kappa * w)) ** (-kappa - 1)
+        norm, _ = integrate.quad(shape, 1.0, np.inf, limit=400, epsrel=1e-11)
+        for X in (0.01, 0.1, 1.0, 10.0, 100.0, 1e3, 1e4):
+            for sth in ((0.8, 0.3) if X == 100.0 else (0.8,)):
+                nu_over_nuc = X * sth * (w * kappa) ** 2      # X = nu / nu_s
+                def n_over(g):
+                    return shape(g) / norm
+                def dn_g2(g):                                  # d/dgamma (n / gamma^2)
+                    q = 1 + (g - 1) / (kappa * w)
+                    b = np.sqrt(g * g - 1)
+                    return (q ** (-kappa - 1) / (g * g * b)
+                            - b / g * (kappa + 1) / (kappa * w) * q ** (-kappa - 2)) / norm
+                def xarg(g):
+                    return 2 * nu_over_nuc / (3 * g * g * sth)
+                lo, hi = np.log(1 + 1e-10), np.log(1e9)
+                gpk = np.sqrt(2 * nu_over_nuc / (3 * sth))
+                pts = sorted(set(np.log(np.clip([gpk / 30, gpk / 3, gpk, 3 * gpk, 30 * gpk,
+                                                 1 + kappa * w], 1 + 1e-9, 1e8))))
+                Ij, _ = integrate.quad(lambda lg: np.exp(lg) * n_over(np.exp(lg))
+                                       * Fsync(xarg(np.exp(lg))), lo, hi, points=pts,
+                                       limit=800, epsrel=1e-9)
+                Ia, _ = integrate.quad(lambda lg: np.exp(lg) * Fsync(xarg(np.exp(lg)))
+                                       * np.exp(2 * lg) * dn_g2(np.exp(lg)), lo, hi, points=pts,
+                                       limit=800, epsrel=1e-9)
+                J = np.sqrt(3) / 2 * Ij
+                # alpha B sin / (n e) with nu = X nu_s and e B = 2 pi m c nu_c:
+                # -sqrt(3) e^2 nu_c sin / (4 nu^2 m c) * Ia * B sin / e
+                #   = -sqrt(3) pi sin^2 / 2 * (nu_c / nu)^2 * Ia
+                A = -np.sqrt(3) * np.pi * sth * sth / 2 / nu_over_nuc ** 2 * Ia
+                rows.append({"w": w, "X": X, "sin_theta": sth, "J": float(J), "A": float(A)})
+                print("kappa w=%g X=%g sth=%g -> J %.6e A %.6e" % (w, X, sth, J, A), flush=True)
+    with open(os.path.join(HERE, "kappa.json"), "w") as fh:
+        json.dump({"kappa": kappa, "rows": rows}, fh, indent=1)
+
+
+def thsynchav_golden():
+    import numpy as np
+    from scipy import integrate, special
+
+    Fsync = _fsync_table()
+    mus, wts = np.polynomial.legendre.leggauss(24)   # cos(theta) on [-1, 1]
+    rows = []
+    for Theta in (3.0, 10.0, 50.0):
+        K2 = special.kv(2, 1.0 / Theta)
+        for xM in (1.0, 10.0, 100.0, 1000.0):
+            nu_over_nuc = xM * 1.5 * Theta ** 2
+            acc = 0.0
+            for mu, wt in zip(mus, wts):
+                sth = np.sqrt(1 - mu * mu)
+                def integrand(lg):
+                    g = np.exp(lg)
+                    beta = np.sqrt(1 - 1 / g ** 2)
+                    return (g * g ** 2 * beta * np.exp(-g / Theta) / (Theta * K2)
+                            * Fsync(2 * nu_over_nuc / (3 * g ** 2 * sth)))
+                gpk = np.sqrt(2 * nu_over_nuc / (3 * sth))
+                pts = sorted(set(np.log([max(1.0 + 1e-9, v) for v in
+                                         (gpk / 10, gpk, gpk * 10, Theta, 10 * Theta)])))
+                I, _ = integrate.quad(integrand, 0.0, np.log(300 * Theta), points=pts,
+                                      limit=400, epsrel=1e-10)
+                acc += 0.5 * wt * np.sqrt(3) * sth / 2 * I
+            rows.append({"Theta": Theta, "x_M": xM, "j_over_ne_e2nuc_c": float(acc)})
+            print("thsynchav Theta=%g x_M=%g -> %.6e" % (Theta, xM, acc), flush=True)
+    with open(os.path.join(HERE, "thsynchav.json"), "w") as fh:
+        json.dump({"rows": rows}, fh, indent=1)
+
+
 if __name__ == "__main__":
     import sys
-    what = sys.argv[1:] or ["kerr", "thsynch"]
+    what = sys.argv[1:] or ["kerr", "thsynch", "kappa", "thsynchav"]
     if "kerr" in what:
         kerr_mks_golden()
     if "thsynch" in what:
         thsynch_golden()
+    if "kappa" in what:
+        kappa_golden()
+    if "thsynchav" in what:
+        thsynchav_golden()

@@ -26,14 +26,14 @@ static inline __host__ void sincospi(double x, double *s, double *c)
 
 using namespace rt;
 
-template <int NF, bool GRAD, int INT>
+template <int NF, bool GRAD, int INT, int EMIS>
 static void finish(const Params &P, const float4 *prims, int ray, Ray<NF, GRAD> &R, double *img,
                    double *dimg, int *cnt)
 {
     const bool dim3 = P.n3 > 1;
     while (ray_alive<NF, GRAD>(P, R)) {
-        bool cont = dim3 ? ray_step<NF, GRAD, true, INT>(P, prims, R)
-                         : ray_step<NF, GRAD, false, INT>(P, prims, R);
+        bool cont = dim3 ? ray_step<NF, GRAD, true, INT, EMIS>(P, prims, R)
+                         : ray_step<NF, GRAD, false, INT, EMIS>(P, prims, R);
         if (!cont)
             break;
     }
@@ -54,7 +54,7 @@ static void finish(const Params &P, const float4 *prims, int ray, Ray<NF, GRAD> 
 // launch); otherwise the split launch's path: approach_ray with near =
 // stopx1 + near_off, the saved ray copied field by field as the trace kernel
 // loads it, then the same loop.
-template <int NF, bool GRAD, int INT>
+template <int NF, bool GRAD, int INT, int EMIS>
 static void run(const Params &P, const float4 *prims, int total, int split, double near_off,
                 double *img, double *dimg, int *cnt)
 {
@@ -62,7 +62,7 @@ static void run(const Params &P, const float4 *prims, int total, int split, doub
         Ray<NF, GRAD> R;
         if (!split) {
             ray_init<NF, GRAD>(P, ray / P.npix, ray % P.npix, R);
-            finish<NF, GRAD, INT>(P, prims, ray, R, img, dimg, cnt);
+            finish<NF, GRAD, INT, EMIS>(P, prims, ray, R, img, dimg, cnt);
             continue;
         }
         Ray<1, false> A, S;
@@ -90,7 +90,7 @@ static void run(const Params &P, const float4 *prims, int total, int split, doub
         R.samples = 0;
         R.terminate = false;
         R.acc.clear();
-        finish<NF, GRAD, INT>(P, prims, ray, R, img, dimg, cnt);
+        finish<NF, GRAD, INT, EMIS>(P, prims, ray, R, img, dimg, cnt);
     }
 }
 
@@ -125,12 +125,24 @@ extern "C" int hc_render_split(const rt_params *p, const rt_grid *g, const float
     int total = ncam * P->npix;
     auto go = [&](auto integ) {
         constexpr int INT = decltype(integ)::value;
+        if (p->emission != RT_EMIS_THSYNCH) {   // image only, like the library
+            if (dimg)
+                return;
+            if (p->emission == RT_EMIS_KAPPA) {
+                if (nf == 1) run<1, false, INT, kKappa>(*P, aos.data(), total, split, near_off, img, dimg, cnt);
+                else run<16, false, INT, kKappa>(*P, aos.data(), total, split, near_off, img, dimg, cnt);
+            } else {
+                if (nf == 1) run<1, false, INT, kThSynchAv>(*P, aos.data(), total, split, near_off, img, dimg, cnt);
+                else run<16, false, INT, kThSynchAv>(*P, aos.data(), total, split, near_off, img, dimg, cnt);
+            }
+            return;
+        }
         if (nf == 1) {
-            if (dimg) run<1, true, INT>(*P, aos.data(), total, split, near_off, img, dimg, cnt);
-            else run<1, false, INT>(*P, aos.data(), total, split, near_off, img, dimg, cnt);
+            if (dimg) run<1, true, INT, kThSynch>(*P, aos.data(), total, split, near_off, img, dimg, cnt);
+            else run<1, false, INT, kThSynch>(*P, aos.data(), total, split, near_off, img, dimg, cnt);
         } else {
-            if (dimg) run<16, true, INT>(*P, aos.data(), total, split, near_off, img, dimg, cnt);
-            else run<16, false, INT>(*P, aos.data(), total, split, near_off, img, dimg, cnt);
+            if (dimg) run<16, true, INT, kThSynch>(*P, aos.data(), total, split, near_off, img, dimg, cnt);
+            else run<16, false, INT, kThSynch>(*P, aos.data(), total, split, near_off, img, dimg, cnt);
         }
     };
     if (p->integrator == RT_INT_RK4)
@@ -156,4 +168,24 @@ extern "C" void hc_shifted_point(double X1, double X2, double e1, double e2, dou
         out[i] = va[i];
         out[5 + i] = vb[i];
     }
+}
+
+// The FP32 emission models of the transfer pass, host build: out = j_nu
+// (cgs), alpha_nu.
+extern "C" void hc_emission(int model, double B, double mu, double Te, double nu, double ne,
+                            double *out)
+{
+    if (model == kThSynch) {
+        EmisF e = emission_f32((float)B, (float)mu, (float)Te, (float)nu, (float)ne);
+        double jx = (double)e.jh * exp(-(double)e.cb);
+        out[0] = kE2C * jx;
+        out[1] = e.jh != 0.f ? jx * (double)e.kap : (e.poison ? NAN : 0.);
+        return;
+    }
+    const KappaConst K = kappa_const();
+    EmisX e = model == kKappa
+                  ? emission_kappa(K, (float)B, (float)mu, (float)Te, (float)nu, (float)ne)
+                  : emission_thsynchav((float)B, (float)Te, (float)nu, (float)ne);
+    out[0] = kE2C * (double)e.jx;
+    out[1] = (model == kKappa || e.jx != 0.f) ? (double)e.alpha : (e.poison ? NAN : 0.);
 }

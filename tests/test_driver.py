@@ -53,6 +53,9 @@ def test_usage_and_argument_errors(driver, tmp_path):
     r = _run(driver, ["model.in", "d", "1e23", "60", "10", "1", "0", "--integrator", "euler"],
              tmp_path)
     assert r.returncode == 2 and "--integrator" in r.stderr
+    r = _run(driver, ["model.in", "d", "1e23", "60", "10", "1", "0", "--emission", "powerlaw"],
+             tmp_path)
+    assert r.returncode == 2 and "--emission" in r.stderr
 
 
 def test_model_in_and_dump_errors(driver, tmp_path):
@@ -161,3 +164,30 @@ def test_driver_integrator_flag(driver, tmp_path):
     for name, text in want.items():
         assert (tmp_path / "output" / name).read_text() == text, name
     assert not np.array_equal(img4, img2)
+
+
+@pytest.mark.gpu
+def test_driver_emission_flag(driver, tmp_path):
+    """--emission kappa renders with the kappa fits: the image equals the
+    library's kappa render and differs from the thermal one."""
+    from raptor_amd import rt
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        d = torus.fishbone_moncrief(48, 48)
+    harm.write_harm_ascii(str(tmp_path / "dump.txt"), d)
+    W, H = 16, 16
+    (tmp_path / "model.in").write_text(MODEL_IN.format(W=W, H=H))
+    r = _run(driver, ["model.in", "dump.txt", "%g" % M_UNIT, "60", "%g" % R_HIGH, "1", "0",
+                      "--emission", "kappa"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    with rt.Tracer(0) as tr:
+        tr.set_params(width=W, height=H, M_unit=M_UNIT, R_high=R_HIGH, emission=rt.EMIS_KAPPA)
+        tr.load_dump(d)
+        imgk, _ = tr.render([60.0], [2.3e11])
+        tr.set_params(width=W, height=H, M_unit=M_UNIT, R_high=R_HIGH)
+        imgt, _ = tr.render([60.0], [2.3e11])
+        jf = tr.jansky_factor()
+    want = _expected_files(tmp_path, imgk[0], W, H, [2.3e11], 60.0, "dump.txt", jf)
+    for name, text in want.items():
+        assert (tmp_path / "output" / name).read_text() == text, name
+    assert not np.array_equal(imgk, imgt)

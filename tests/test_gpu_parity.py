@@ -240,6 +240,95 @@ def test_bad_integrator_rejected(tracer):
         tracer.set_params(width=8, height=8, M_unit=M_UNIT, R_high=R_HIGH, integrator=3)
 
 
+@pytest.mark.parametrize("emission", [rt.EMIS_THSYNCHAV, rt.EMIS_KAPPA])
+def test_other_emission_models(tracer, dump128, emission):
+    """The angle-averaged thermal fit and the kappa fits (j_nu.c:28-106,
+    157-167, which emission_coeff does not call; chosen here by
+    rt_params.emission): images at one frequency and at three (the 16-wide
+    kernel) against the oracle's, with and without the approach kernel
+    (bit-identical)."""
+    W = 48
+    tracer.set_params(width=W, height=W, M_unit=M_UNIT, R_high=R_HIGH, emission=emission)
+    tracer.load_dump(dump128)
+    P = O.params(dump128.grid, width=W, height=W, inclination=INCL, M_unit=M_UNIT,
+                 R_high=R_HIGH, emission=emission)
+    try:
+        for freqs in ([FREQ], [8.6e10, FREQ, 6.9e11]):
+            imgs = []
+            for approach in (rt.APPROACH_NONE, 100):
+                tracer.set_launch_config(approach=approach)
+                img, _ = tracer.render([INCL], freqs)
+                imgs.append(img[0])
+            assert np.array_equal(imgs[0], imgs[1])
+            ora, gate = O.render(P, dump128.prims, freqs, gate=True)
+            for f in range(len(freqs)):
+                r = _report("emission %d nu %.3g" % (emission, freqs[f]), imgs[0][:, f],
+                            ora[:, f])
+                assert ora[:, f].max() > 0
+                assert abs(imgs[0][:, f].sum() / ora[:, f].sum() - 1) < FLUX_TOL
+                assert r[~(gate[:, f] < GATE_EPS)].max() < PIX_TOL
+    finally:
+        tracer.set_launch_config()
+
+
+def test_other_emission_models_3d(tracer, dump3d_small):
+    """The kappa fits on a 3D dump (trilinear sampler kernels)."""
+    W = 24
+    tracer.set_params(width=W, height=W, M_unit=M_UNIT, R_high=R_HIGH, emission=rt.EMIS_KAPPA)
+    tracer.load_dump(dump3d_small)
+    P = O.params(dump3d_small.grid, width=W, height=W, inclination=INCL, M_unit=M_UNIT,
+                 R_high=R_HIGH, emission=O.KAPPA)
+    img, _ = tracer.render([INCL], [FREQ])
+    ora, gate = O.render(P, dump3d_small.prims, [FREQ], gate=True)
+    r = _report("kappa 3d", img[0, :, 0], ora[:, 0])
+    assert abs(img[0, :, 0].sum() / ora[:, 0].sum() - 1) < FLUX_TOL
+    assert r[~(gate[:, 0] < GATE_EPS)].max() < PIX_TOL
+
+
+def test_emission_model_errors(tracer, dump128):
+    with pytest.raises(rt.RtError, match="emission"):
+        tracer.set_params(width=8, height=8, M_unit=M_UNIT, R_high=R_HIGH, emission=3)
+    tracer.set_params(width=8, height=8, M_unit=M_UNIT, R_high=R_HIGH, emission=rt.EMIS_KAPPA)
+    tracer.load_dump(dump128)
+    with pytest.raises(rt.RtError, match="thermal synchrotron"):
+        tracer.render([INCL], [FREQ], grad=True)
+    with pytest.raises(rt.RtError, match="emission model"):
+        tracer.probe(rt.PROBE_EMISSION_MODEL, [[10.0, 0.5, 10.0, 2.3e11, 1e5, 7.0]])
+
+
+def test_probe_emission_models(tracer):
+    """Device FP32 fits of every emission model against the FP64 oracle over
+    plasma states from the Theta_e floor to 300, including the gates (nu_p
+    floor, |cos pitch| = 1, |z| = 1 in the kappa absorptivity)."""
+    rng = np.random.default_rng(9)
+    n = 600
+    for model in (O.THSYNCH, O.THSYNCHAV, O.KAPPA):
+        q = np.column_stack([10 ** rng.uniform(-1, 3, n), rng.uniform(-1, 1, n),
+                             10 ** rng.uniform(-3, 2.5, n), 10 ** rng.uniform(9, 13, n),
+                             10 ** rng.uniform(2, 8, n), np.full(n, float(model))])
+        q[:10, 2] = 1e-20
+        q[10:20, 2] = 1.0 / 3.5
+        q[20:30, 3] = 1.0
+        q[30:40, 1] = 1.0
+        out = tracer.probe(rt.PROBE_EMISSION_MODEL, q)
+        for i in range(n):
+            j, a = O.emission(model, *q[i, :5])
+            nuc = 4.8032068e-10 * q[i, 0] / (2 * np.pi * 9.1093826e-28 * 2.99792458e10)
+            # exp(-X^(1/3)) (thermal) and exp(-1.8899 x_M^(1/3)) (averaged) multiply
+            # the FP32 relative error of the cube root by that exponent
+            X = q[i, 3] / max(nuc * q[i, 2] ** 2, 1e-300)
+            tol = 5e-4 + 2e-6 * np.cbrt(X)
+            x = 6.6260693e-27 * q[i, 3] / (q[i, 2] * 9.1093826e-28 * 2.99792458e10 ** 2)
+            tola = tol + (4e-16 / x if model != O.KAPPA else 0.0)
+            for got, ref, t in ((out[i, 0], j, tol), (out[i, 1], a, tola)):
+                if np.isnan(ref) or np.isnan(got):
+                    assert np.isnan(ref) and np.isnan(got), (model, q[i], got, ref)
+                elif ref == 0.0 or got == 0.0:
+                    assert abs(got - ref) <= 1e-30, (model, q[i], got, ref)
+                else:
+                    assert got == pytest.approx(ref, rel=t), (model, q[i], got, ref)
+
+
 # ---- building blocks ------------------------------------------------------
 
 

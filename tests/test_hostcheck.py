@@ -32,7 +32,7 @@ def _lib():
 
 
 def _render(dump, W, incl, freqs, grad=False, R_high=R_HIGH, stepsize=0.03, integrator=0,
-            near_off=None):
+            near_off=None, emission=0):
     L = _lib()
     g = dump.grid
     grid = rt.RtGrid(n1=g.n1, n2=g.n2, n3=g.n3, startx1=g.startx1, startx2=g.startx2,
@@ -40,7 +40,7 @@ def _render(dump, W, incl, freqs, grad=False, R_high=R_HIGH, stepsize=0.03, inte
                      R0=g.R0)
     p = rt.RtParams(MBH=8.9505e39, M_unit=M_UNIT, R_high=R_high, R_low=1.0, width=W, height=W,
                     cam_size_x=40.0, cam_size_y=40.0, stepsize=stepsize, max_steps=100000,
-                    integrator=integrator)
+                    integrator=integrator, emission=emission)
     prims = np.ascontiguousarray(dump.prims, np.float32)
     incl = np.ascontiguousarray(incl, float)
     fr = np.ascontiguousarray(freqs, float)
@@ -175,3 +175,55 @@ def test_split_approach_equals_one_loop(dump128, integrator):
                      near_off=-100.0)
     for a, b in zip(forced, ref):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("model", [O.THSYNCHAV, O.KAPPA])
+def test_host_build_emission_models_unit(model):
+    """The FP32 fits of the other emission models (host build) against the
+    oracle's FP64 restatement over the plasma states a torus sample meets and
+    beyond (Theta_e from the 1e-20 floor to 300, B, n_e, nu_p over decades)."""
+    L = _lib()
+    L.hc_emission.argtypes = [ctypes.c_int] + [ctypes.c_double] * 5 + [ctypes.c_void_p]
+    rng = np.random.default_rng(7 + model)
+    out = np.zeros(2)
+    n = 4000
+    q = np.column_stack([10 ** rng.uniform(-1, 3, n), rng.uniform(-1, 1, n),
+                         10 ** rng.uniform(-3, 2.5, n), 10 ** rng.uniform(9, 13, n),
+                         10 ** rng.uniform(2, 8, n)])
+    q[:20, 2] = 1e-20
+    q[20:40, 2] = 1.0 / 3.5          # |z| = 1 in the kappa absorptivity
+    q[40:60, 3] = 1.0                # nu_p floor
+    for row in q:
+        L.hc_emission(model, *row, ctypes.c_void_p(out.ctypes.data))
+        j, a = O.emission(model, *row)
+        # Kirchhoff's absorptivity: the oracle's Planck factor exp(x) - 1
+        # (j_nu.c:190) loses ~1e-16/x, the kernel uses expm1
+        x = 6.6260693e-27 * row[3] / (row[2] * 9.1093826e-28 * 2.99792458e10 ** 2)
+        tola = 2e-4 + (4e-16 / x if model == O.THSYNCHAV else 0.0)
+        for got, ref, tol in ((out[0], j, 2e-4), (out[1], a, tola)):
+            if np.isnan(ref) or np.isnan(got):
+                assert np.isnan(ref) and np.isnan(got), (row, got, ref)
+            elif ref == 0.0 or got == 0.0:
+                # FP32 underflow against FP64 values far below any pixel's noise
+                assert abs(got - ref) <= 1e-30, (row, got, ref)
+            else:
+                assert got == pytest.approx(ref, rel=tol), (row, got, ref)
+
+
+@pytest.mark.parametrize("model", [O.THSYNCHAV, O.KAPPA])
+def test_host_build_emission_models_image(dump128, model):
+    """Whole images with the other emission models: the tracer's physics on
+    the CPU against the oracle (1e-3 per pixel off the tau gate, 1e-4 in
+    flux), and different from the thermal image."""
+    W = 24
+    img, _, cnt = _render(dump128, W, [INCL], [FREQ], emission=model)
+    P = O.params(dump128.grid, width=W, height=W, inclination=INCL, M_unit=M_UNIT,
+                 R_high=R_HIGH, emission=model)
+    oimg, ocnt, gate = O.render(P, dump128.prims, [FREQ], counters=True, gate=True)
+    assert oimg.max() > 0
+    near = gate[:, 0] < 1e-4
+    assert rel_err(img[:, 0], oimg[:, 0])[~near].max() < 1e-3
+    assert abs(img[:, 0].sum() / oimg[:, 0].sum() - 1) < 1e-4
+    th = O.render(O.params(dump128.grid, width=W, height=W, inclination=INCL, M_unit=M_UNIT,
+                           R_high=R_HIGH), dump128.prims, [FREQ])
+    assert abs(th[:, 0].sum() / oimg[:, 0].sum() - 1) > 1e-3

@@ -369,6 +369,126 @@ def test_thsynch_low_frequency_slope_and_gates():
     assert lib.or_j_thsynch(2 * B, 0.6, Th, 1e11, 2 * ne) > 0
 
 
+# ------------------------------------------------ other emission models
+
+
+def test_hyp2f1_against_scipy_and_closed_forms():
+    """or_hyp2f1 (Pfaff's transformation for -1 <= z <= 0, the 1/(1-z)
+    continuation below) against scipy.special.hyp2f1 at the kappa model's
+    parameters (a, b, c) = (kappa - 1/3, kappa + 1, kappa + 2/3) over the
+    z = -kappa Theta_e it meets, and at other parameters with b - a not an
+    integer; and the closed forms 2F1(a, b; b; z) = (1-z)^-a and
+    2F1(1, 1; 2; z) = -ln(1-z)/z (a swapped argument or a wrong
+    transformation is off by far more than 1e-12)."""
+    from scipy import special
+    lib = O.lib()
+    k = 3.5
+    for z in -np.concatenate([np.logspace(-4, 3.5, 40), [1.0, 0.999, 1.001]]):
+        assert lib.or_hyp2f1(k - 1 / 3, k + 1, k + 2 / 3, z) == pytest.approx(
+            special.hyp2f1(k - 1 / 3, k + 1, k + 2 / 3, z), rel=1e-12), z
+    for a, b, c in ((0.5, 1.7, 2.3), (1.2, 0.4, 3.1), (2.5, 0.25, 4.5), (3.0, 5.5, 4.0)):
+        for z in (-0.05, -0.5, -0.97, -1.3, -7.0, -80.0):
+            assert lib.or_hyp2f1(a, b, c, z) == pytest.approx(special.hyp2f1(a, b, c, z),
+                                                              rel=1e-11), (a, b, c, z)
+    for z in (-0.3, -2.0, -40.0):
+        assert lib.or_hyp2f1(0.7, 1.9, 1.9, z) == pytest.approx((1 - z) ** -0.7, rel=1e-13)
+    for z in (-0.2, -0.9):
+        assert lib.or_hyp2f1(1.0, 1.0, 2.0, z) == pytest.approx(-math.log(1 - z) / z, rel=1e-13)
+
+
+def _kappa_fit(r):
+    lib = O.lib()
+    B, ne = 30.0, 1e6
+    w, X, sth = r["w"], r["X"], r["sin_theta"]
+    mu = math.sqrt(1 - sth * sth)
+    nuc = E * B / (2 * math.pi * ME * C)
+    nu = X * nuc * sth * (w * 3.5) ** 2
+    J = lib.or_j_kappa(nu, ne, w, B, mu) / (ne * E * E * nuc * sth / C)
+    A = lib.or_a_kappa(nu, ne, w, B, mu) * B * sth / (ne * E)
+    return J, A
+
+
+def test_kappa_fits_against_distribution_integrals():
+    """The kappa = 3.5 emissivity and absorptivity fits (j_nu.c:28-106,
+    Pandya et al. 2016) against integrals over the kappa distribution of the
+    single-electron spectrum (tests/golden/kappa.json; the absorptivity
+    integral reproduces Kirchhoff's law for a thermal distribution to 3e-4).
+    Emissivity: within 6% for widths w = Theta_e >= 10, the fits' regime,
+    and 50% at w = 1.  Absorptivity: the fit, as the reference writes it,
+    runs from 0.94 of the integral at X = nu/nu_s = 0.01 (the A_low term with
+    the hypergeometric factor) to 0.48 at X = 1e4 (the A_high term); the band
+    [0.45, 0.96] and the monotone fall in X catch a wrong exponent, a dropped
+    2F1 or Gamma factor, or swapped terms (each moves the ratio by a factor of
+    2 or more, or by orders of magnitude across X).  Both depend on X alone,
+    not on the pitch angle, like the integrals."""
+    rows = json.load(open(os.path.join(GOLD, "kappa.json")))["rows"]
+    by_w = {}
+    for r in rows:
+        J, A = _kappa_fit(r)
+        rj, ra = J / r["J"], A / r["A"]
+        if r["w"] >= 10:
+            assert rj == pytest.approx(1.0, abs=0.06), r
+            assert 0.45 <= ra <= 0.96, r
+            if r["sin_theta"] == 0.8:
+                by_w.setdefault(r["w"], []).append((r["X"], ra))
+        elif r["w"] == 1.0:
+            assert 0.5 <= rj <= 1.5 and 0.4 <= ra <= 0.8, r
+    for w, pts in by_w.items():
+        ra = [q for _, q in sorted(pts)]
+        assert all(b < a for a, b in zip(ra, ra[1:])), (w, ra)
+    same = [r for r in rows if r["X"] == 100.0]
+    for w in {r["w"] for r in same}:
+        a, b = [_kappa_fit(r) for r in same if r["w"] == w]
+        assert a == pytest.approx(b, rel=1e-12)
+
+
+def test_kappa_gates():
+    lib = O.lib()
+    B, ne, w = 30.0, 1e6, 5.0
+    assert lib.or_j_kappa(1.0, ne, w, B, 0.5) == 0.0          # nu_p floor
+    assert lib.or_j_kappa(1e11, ne, w, B, 1.0) == 0.0         # |cos pitch| = 1
+    assert lib.or_a_kappa(1e11, ne, w, B, -1.0) == 0.0
+    assert lib.or_a_kappa(1e11, ne, 1.0 / 3.5, B, 0.5) == 0.0  # |z| = 1 (j_nu.c:79)
+    assert lib.or_j_kappa(1e11, ne, 1.0 / 3.5, B, 0.5) > 0.0
+    # emissivity and absorptivity scale with n_e
+    assert lib.or_j_kappa(1e11, 2 * ne, w, B, 0.5) == pytest.approx(
+        2 * lib.or_j_kappa(1e11, ne, w, B, 0.5), rel=1e-14)
+
+
+def test_thsynchav_against_angle_averaged_integral():
+    """The angle-averaged thermal fit (j_nu.c:157-167, Dexter 2009) times e^2/c
+    against the thermal emissivity integral averaged over isotropic pitch
+    angles (tests/golden/thsynchav.json): within 3% for Theta_e >= 3 and
+    x_M = 1..1000 (a dropped 2 sqrt(3), a wrong x_M power or the Mahadevan
+    coefficients mistyped are off by 20% or more)."""
+    rows = json.load(open(os.path.join(GOLD, "thsynchav.json")))["rows"]
+    lib = O.lib()
+    B, ne = 30.0, 1e6
+    nuc = E * B / (2 * math.pi * ME * C)
+    for r in rows:
+        nu = r["x_M"] * 1.5 * nuc * r["Theta"] ** 2
+        j = lib.or_j_thsynchav(B, r["Theta"], nu, ne) * (E * E / C)
+        assert j / (ne * E * E * nuc / C) / r["j_over_ne_e2nuc_c"] == pytest.approx(
+            1.0, abs=0.03), r
+
+
+def test_emission_model_composition():
+    """or_emission: model 0 is the path's thermal fit with Kirchhoff's
+    absorptivity; model 1 the angle-averaged fit in cgs with Kirchhoff's;
+    model 2 the kappa emissivity and absorptivity fits."""
+    lib = O.lib()
+    q = (30.0, 0.4, 12.0, 2.3e11, 1e6)
+    j0, a0 = O.emission(O.THSYNCH, *q)
+    assert j0 == lib.or_j_thsynch(*q) and a0 == lib.or_a_th(j0, q[3], q[2])
+    j1, a1 = O.emission(O.THSYNCHAV, *q)
+    assert j1 == pytest.approx(lib.or_j_thsynchav(q[0], q[2], q[3], q[4]) * (E * E / C),
+                               rel=1e-15)
+    assert a1 == lib.or_a_th(j1, q[3], q[2])
+    j2, a2 = O.emission(O.KAPPA, *q)
+    assert j2 == lib.or_j_kappa(q[3], q[4], q[2], q[0], q[1])
+    assert a2 == lib.or_a_kappa(q[3], q[4], q[2], q[0], q[1])
+
+
 def test_kirchhoff_and_planck_limits():
     """alpha = j / B_nu with B_nu = 2 h nu^3/c^2 / (exp(h nu/k T) - 1): the
     reference's Planck function omits h and e2_ch/e2_c restores it."""
