@@ -270,17 +270,28 @@ constexpr bool three_waves()
 template <int NF, bool GRAD, bool DIM3, int INT, bool RESUME, int EMIS>
 __device__ __forceinline__ void trace_rays(const Params &P, const TraceArgs &A)
 {
+    // resumed RK2 rays overlap each transfer sample with the next step's
+    // first acceleration (ray_step_rk2_overlap), carried in A0
+    constexpr bool kOverlap = RESUME && INT == kRK2;
     Ray<NF, GRAD> R;
+    double A0[4] = {0., 0., 0., 0.};
     int ray = 0, slot = 0;
     bool active = claim<RESUME>(P, A, ray, slot);
     bool exhausted = !active;
-    if (active)
+    if (active) {
         start_ray<RESUME>(P, A, ray, slot, R);
+        if (kOverlap)
+            geodesic_accel(R.pt, P.a, P.hslope, R.y + 4, A0);
+    }
     while (true) {
         if (active) {
             bool cont = ray_alive<NF, GRAD>(P, R);
-            if (cont)
-                cont = ray_step<NF, GRAD, DIM3, INT, EMIS>(P, A.prims, R);
+            if (cont) {
+                if constexpr (kOverlap)
+                    cont = ray_step_rk2_overlap<NF, GRAD, DIM3, EMIS>(P, A.prims, R, A0);
+                else
+                    cont = ray_step<NF, GRAD, DIM3, INT, EMIS>(P, A.prims, R);
+            }
             if (!cont) {
                 write_ray<NF, GRAD, DIM3>(P, A, ray, R);
                 active = false;
@@ -304,6 +315,8 @@ __device__ __forceinline__ void trace_rays(const Params &P, const TraceArgs &A)
             if (!active) {
                 if (claim<RESUME>(P, A, ray, slot)) {
                     start_ray<RESUME>(P, A, ray, slot, R);
+                    if (kOverlap)
+                        geodesic_accel(R.pt, P.a, P.hslope, R.y + 4, A0);
                     active = true;
                 } else {
                     exhausted = true;

@@ -337,16 +337,16 @@ __host__ __device__ inline double stepsize(const double y[8], double eps)
 // y's point on entry and of the new point on exit.  The time coordinate
 // X^0 is not advanced: nothing on the path reads it (the metric is
 // stationary; the reference only prints it in its debug geodesic dump).
-__host__ __device__ inline void rk2_step(double y[8], Point &pt, double dl, double a, double h)
+__host__ __device__ inline void rk2_step_from(double y[8], Point &pt, double dl, double a,
+                                              double h, const double A0[4])
 {
     double A[4], ym[8];
-    geodesic_accel(pt, a, h, y + 4, A);
     const double hdl = 0.5 * dl;   // exact; one FMA per component
     ym[0] = y[0];
     for (int i = 1; i < 4; i++)
         ym[i] = y[i] + hdl * y[4 + i];
     for (int i = 0; i < 4; i++)
-        ym[4 + i] = y[4 + i] + hdl * A[i];
+        ym[4 + i] = y[4 + i] + hdl * A0[i];
     Point pm = shifted_point(pt, ym[1], ym[2], ym[1] - y[1], ym[2] - y[2], h);
     geodesic_accel(pm, a, h, ym + 4, A);
     const double X1o = y[1], X2o = y[2];
@@ -355,6 +355,16 @@ __host__ __device__ inline void rk2_step(double y[8], Point &pt, double dl, doub
     for (int i = 0; i < 4; i++)
         y[4 + i] = y[4 + i] + dl * A[i];
     pt = shifted_point(pt, y[1], y[2], y[1] - X1o, y[2] - X2o, h);
+}
+
+// rk2_step_from with the start point's acceleration A0 given: the same
+// arithmetic, so a caller that has A0 already (computed beside the previous
+// step's transfer sample) gets the same bits.
+__host__ __device__ inline void rk2_step(double y[8], Point &pt, double dl, double a, double h)
+{
+    double A0[4];
+    geodesic_accel(pt, a, h, y + 4, A0);
+    rk2_step_from(y, pt, dl, a, h, A0);
 }
 
 // The steppers of integrator.c:258-263, chosen by int_method
@@ -1065,6 +1075,48 @@ __host__ __device__ inline bool approach_ray(const Params &P, int cam, int pix, 
             return false;
     }
     return false;
+}
+
+// ray_advance for RK2 with the start point's acceleration given.
+template <int NF, bool GRAD>
+__host__ __device__ inline bool ray_advance_from(const Params &P, Ray<NF, GRAD> &R, double &dl,
+                                                 const double A0[4])
+{
+    const int sgn = (R.y[6] > 0.) - (R.y[6] < 0.);
+    if (R.thetadot_sign * sgn < 0 && R.steps > 2)
+        R.turns += 1;
+    R.thetadot_sign = sgn;
+    if ((R.beta_sign < 0 && R.turns > kMaxOrder) || (R.beta_sign > 0 && R.turns > kMaxOrder + 1))
+        R.terminate = true;
+    dl = stepsize(R.y, P.stepsize);
+    rk2_step_from(R.y, R.pt, dl, P.a, P.hslope, A0);
+    return !(R.y[1] > P.stopx1 && R.y[5] < 0.);
+}
+
+// One RK2 iteration of integrate_geodesic's loop for rays inside the grid
+// (the persistent kernel resuming saved rays, whose steps nearly all sample):
+// the fluid sample of the new point is evaluated whether or not the point is
+// inside the grid (its result is used only if it is) so that it shares a
+// basic block with the next step's first acceleration, A0 on return, and the
+// two independent chains interleave.  The same arithmetic as ray_step.
+template <int NF, bool GRAD, bool DIM3, int EMIS = kThSynch>
+__host__ __device__ inline bool ray_step_rk2_overlap(const Params &P,
+                                                     const float4 *__restrict__ prims,
+                                                     Ray<NF, GRAD> &R, double A0[4])
+{
+    double dl;
+    if (!ray_advance_from(P, R, dl, A0))
+        return false;
+    const bool inside = in_grid(P, R.y);
+    Fluid F;
+    const bool sampled = fluid_sample<DIM3>(P, prims, R.pt, R.y, F);
+    geodesic_accel(R.pt, P.a, P.hslope, R.y + 4, A0);
+    if (inside && sampled) {
+        R.samples++;
+        transfer_sample<NF, GRAD, EMIS>(P, F, fabs(dl), R.acc);
+    }
+    R.steps++;
+    return true;
 }
 
 // Host set-up.
