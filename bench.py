@@ -149,8 +149,8 @@ def main():
     # of at least 2^20 rays, then the persistent kernel resuming them
     launch = tr.get_launch_config()
     split = launch["approach"] > 0 or (launch["approach"] == 0 and npix >= 1 << 20)
-    kernels = ("approach_kernel<0> + trace_kernel_3w<1,false,false,0,true>" if split
-               else "trace_kernel_3w<1,false,false,0,false>")
+    kernels = ("approach_kernel<0> + trace_kernel_3w<1,false,false,0,true,0>" if split
+               else "trace_kernel_3w<1,false,false,0,false,0>")
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "trace_kernel_traffic.json")
     if W == 1024 and os.path.exists(tfile):

@@ -70,6 +70,42 @@ def test_ragged_sizes_and_inclinations(tracer, dump128, W, H, incl):
     assert r.max() < PIX_TOL
 
 
+def test_maximum_batch_cameras_and_frequencies(tracer, dump128):
+    """The largest batch one call takes: 64 inclinations x 16 frequencies
+    (10 to 2000 GHz) of a small image, every camera and frequency against the
+    oracle's, with and without the approach kernel; one more camera or
+    frequency is refused."""
+    W = 8
+    P = _setup(tracer, dump128, W)
+    incl = list(np.linspace(2.0, 178.0, 64))
+    freqs = list(1e10 * 200.0 ** (np.arange(16) / 15.0))
+    img, spec = tracer.render(incl, freqs)
+    assert img.shape == (64, W * W, 16)
+    tracer.set_launch_config(approach=100)
+    try:
+        img2, _ = tracer.render(incl, freqs)
+    finally:
+        tracer.set_launch_config()
+    assert np.array_equal(img, img2)
+    worst = 0.0
+    for c in range(0, 64, 7):
+        Pc = O.params(dump128.grid, width=W, height=W, inclination=incl[c], M_unit=M_UNIT,
+                      R_high=R_HIGH)
+        ora, gate = O.render(Pc, dump128.prims, freqs, gate=True)
+        for f in range(16):
+            if ora[:, f].max() <= 0:
+                assert np.all(img[c, :, f] == 0)
+                continue
+            r = rel_err(img[c, :, f], ora[:, f])
+            worst = max(worst, r[~(gate[:, f] < GATE_EPS)].max())
+            assert spec[c, f] == pytest.approx(img[c, :, f].sum(), rel=1e-13)
+    assert worst < PIX_TOL
+    with pytest.raises(rt.RtError, match="n_cam"):
+        tracer.render(incl + [60.0], [FREQ])
+    with pytest.raises(rt.RtError, match="n_freq"):
+        tracer.render([60.0], freqs + [3e12])
+
+
 def test_ray_counters_match_oracle(tracer, dump128):
     W = 32
     P = _setup(tracer, dump128, W)
