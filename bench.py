@@ -255,6 +255,32 @@ def main():
     print(json.dumps(result), flush=True)
 
 
+def _roofline(tr, incl, freqs, kernel_ms, grad=False):
+    """Algorithmic FLOP rate of a render whose kernels took kernel_ms, from a
+    counters pass (per-ray steps and samples) of the same launch."""
+    from raptor_amd import flops
+    n = len(incl) * tr.width * tr.height
+    d_img = tr.alloc(n * len(freqs) * 8)
+    d_dimg = tr.alloc(n * len(freqs) * 16) if grad else None
+    d_cnt = tr.alloc(n * 8)
+    try:
+        tr.render_device(incl, freqs, d_img, d_dimg, d_cnt)
+        tr.synchronize()
+        cnt = tr.d2h(d_cnt, (n, 2), np.int32)
+    finally:
+        tr.free(d_img)
+        if d_dimg:
+            tr.free(d_dimg)
+        tr.free(d_cnt)
+    f64, f32 = flops.image_flops_split(cnt[:, 0], cnt[:, 1], len(freqs), grad=grad)
+    peak = flops.mix_peak(f64, f32, FP64_PEAK_TFLOPS, FP32_PEAK_TFLOPS)
+    achieved = (f64 + f32) / (kernel_ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "alg_flops_per_launch": f64 + f32,
+            "rk2_steps_per_launch": int(cnt[:, 0].sum()),
+            "samples_per_launch": int(cnt[:, 1].sum()), "kernel_ms": kernel_ms}
+
+
 def _timed(tr, fn, steps):
     """Mean kernel ms (HIP events) and wall ms per call of fn()."""
     kms = []
@@ -304,6 +330,8 @@ def bench_c2(args):
     b = tr.d2h(d_b, (len(incl), npix, len(freqs)), np.float64)
     s = tr.d2h(d_s, (len(incl), len(freqs), npix), np.float64)
     same = bool(np.array_equal(b, np.transpose(s, (0, 2, 1))))
+    tr.set_launch_config(**tune["config"])
+    roof = _roofline(tr, incl, freqs, kb)
     g.destroy()
     tr.free(d_b)
     tr.free(d_s)
@@ -315,7 +343,7 @@ def bench_c2(args):
         "config": dict(workload="harm2d-torus-256x256_img512x512_6incl_8freq", dump="256x256x1",
                        image="512x512", inclinations=incl, freqs_hz=freqs),
         "batched": {"kernel_ms": kb, "wall_ms": wb, "launch_config": tune["config"],
-                    "images_per_s": len(incl) * len(freqs) / (wb * 1e-3)},
+                    "images_per_s": len(incl) * len(freqs) / (wb * 1e-3), "roofline": roof},
         "per_image_graph": {"launches": len(incl) * len(freqs), "kernel_ms": ks,
                             "wall_ms": ws,
                             "images_per_s": len(incl) * len(freqs) / (ws * 1e-3)},
@@ -350,7 +378,8 @@ def bench_3d(args):
            "config": dict(workload="harm3d-torus-192x96x96_img%dx%d_230GHz_i60" % (W, W),
                           dump="192x96x96", image="%dx%d" % (W, W),
                           prims_MB=dump.prims.size * 4 / 1e6),
-           "image": {"kernel_ms": ki, "wall_ms": wi, "images_per_s": 1e3 / wi}}
+           "image": {"kernel_ms": ki, "wall_ms": wi, "images_per_s": 1e3 / wi,
+                     "roofline": _roofline(tr, incl, freq, ki)}}
     if args.workload == "c4":
         tr.render_device(incl, freq, d_img, d_dimg)
         tr.synchronize()
@@ -386,6 +415,7 @@ def bench_3d(args):
                            "sum_rel": float(fds[0][smooth].sum() / a[smooth].sum() - 1)}
         out["grad"] = {"kernel_ms": kg, "wall_ms": wg, "image_plus_grad_per_s": 1e3 / wg,
                        "ratio_vs_image": wi / wg, "fd_check": stats,
+                       "roofline": _roofline(tr, incl, freq, kg, grad=True),
                        "fd_note": "GPU tangents vs central differences (step 1e-3) of the "
                                   "FP64 oracle on 2048 sampled pixels; pixels whose quotient "
                                   "changes between steps 1e-3 and 5e-4 (gate flips) excluded"}

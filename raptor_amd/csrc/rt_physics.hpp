@@ -129,6 +129,23 @@ __host__ inline float fexp2(float x) { return exp2f(x); }
 __host__ inline float flog2(float x) { return log2f(x); }
 __host__ __device__ inline float fexp(float x) { return fexp2(1.4426950408889634f * x); }
 
+// e^x - 1 for the Planck factor: the Taylor series to x^6/6! for |x| <= 1/4
+// (truncation below x^6/7! < 5e-8 relative) and exp(x) - 1 beyond (the
+// single-instruction exp2's ~1 ulp, at most 5.4e-7 relative at |x| = 1/4);
+// 12 instructions instead of the library's 25.
+__host__ __device__ inline float fexpm1(float x)
+{
+    float p = 1.f / 720.f;
+    p = p * x + 1.f / 120.f;
+    p = p * x + 1.f / 24.f;
+    p = p * x + 1.f / 6.f;
+    p = p * x + 0.5f;
+    p = p * x + 1.f;
+    const float t = x * p;
+    const float e = fexp(x) - 1.f;
+    return fabsf(x) <= 0.25f ? t : e;
+}
+
 // FP64 reciprocal for the geodesic step: the hardware estimate refined by two
 // Newton steps r <- r + r (1 - x r), each of which doubles the correct bits,
 // so the result is within about 1 ulp of 1/x (an IEEE division's scaling and
@@ -724,7 +741,7 @@ __host__ __device__ inline EmisF emission_f32(float B, float mu, float Te, float
     const float inu = frcp(nu_p);
     const float x = (float)(kPlanck / (kElectronMass * kC * kC)) * nu_p * frcp(Te);
     const float cn = (float)kC * inu;
-    e.kap = (float)(0.5 * kE2CH) * expm1f(x) * (cn * cn) * inu;
+    e.kap = (float)(0.5 * kE2CH) * fexpm1(x) * (cn * cn) * inu;
     return e;
 }
 
@@ -832,7 +849,7 @@ __host__ __device__ inline EmisX emission_thsynchav(float B, float Te, float nu_
         // radiative_transfer.c:24-26, where this fit has no gate)
         e.alpha = x < 1.1102230246251565e-16f
                       ? 0.f
-                      : e.jx * ((float)(0.5 * kE2CH) * expm1f(x) * (cn * cn) * inu);
+                      : e.jx * ((float)(0.5 * kE2CH) * fexpm1(x) * (cn * cn) * inu);
     } else {
         double T = (double)Te * kElectronMass * kC * kC / kBoltzmann;
         e.poison = kPlanck * (double)nu_p / (kBoltzmann * T) > 709.782712893384;
@@ -917,7 +934,7 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
             float dlnB = 0.5f * (F.B - 1e-20f) * frcp(F.B);
             float dlnT = F.dthetae * frcp(F.thetae);
             float x = (float)(kPlanck / (kElectronMass * kC * kC)) * nu_p * frcp(F.thetae);
-            float dlnBnu = x * frcp(-expm1f(-x));
+            float dlnBnu = x * frcp(-fexpm1(-x));
             dlnj0 = 1.f + dlnB * (1.f - Dx);
             dlnj1 = -2.f * Dx * dlnT;
             dlna1 = dlnj1 - dlnBnu * dlnT;
