@@ -102,6 +102,7 @@ struct Params {
     double freq[kMaxFreq];
     // FP32 copies for the transfer
     float fnu[kMaxFreq];
+    float finu[kMaxFreq];        // 1 / freq, rounded once from FP64
     float fNe_unit, fB_unit, fiB_unit, fR_high, fR_low, fa, fpi_h;   // fpi_h = (1 - h) pi
     int emission;                // kThSynch, kThSynchAv or kKappa
     KappaConst kc;
@@ -709,12 +710,16 @@ __host__ __device__ inline bool fluid_sample(const Params &P, const float4 *__re
 // jh == 0 marks the reference's zero-emissivity gates.  When j is zero and the
 // reference's FP64 exp(x) overflows (x > 709.78, only for Theta_e below
 // ~1e-11) its absorptivity is 0/0 = NaN; `poison` reports that case.
+// X^(1/2) and X^(1/3) are powers of s6 = X^(1/6), one exp2 for all three
+// roots; inu = 1 / nu_p comes from the caller, which forms it as (1/g)(1/nu)
+// from a per-sample and a per-frequency reciprocal.
 struct EmisF {
-    float jh, cb, kap, X;
+    float jh, cb, kap, X, s6;
     bool poison;
 };
 
-__host__ __device__ inline EmisF emission_f32(float B, float mu, float Te, float nu_p, float ne)
+__host__ __device__ inline EmisF emission_f32(float B, float mu, float Te, float nu_p, float ne,
+                                              float inu)
 {
     EmisF e;
     const float sth = fsqrt(1.f - mu * mu);
@@ -727,30 +732,36 @@ __host__ __device__ inline EmisF emission_f32(float B, float mu, float Te, float
         e.jh = 0.f;
         e.cb = 0.f;
         e.kap = 0.f;
+        e.s6 = 0.f;
         if (!(Te > 1e-4f)) {
             double T = (double)Te * kElectronMass * kC * kC / kBoltzmann;
             e.poison = kPlanck * (double)nu_p / (kBoltzmann * T) > 709.782712893384;
         }
         return e;
     }
-    const float lx = flog2(X);
-    const float cb = fexp2(lx * (1.f / 3.f));                        // X^(1/3)
-    const float ff = fexp2(0.5f * lx) + 1.8877486253633868f * fexp2(lx * (1.f / 6.f));
+    const float s6 = fexp2(flog2(X) * (1.f / 6.f));
+    const float cb = s6 * s6;                                         // X^(1/3)
+    const float ff = cb * s6 + 1.8877486253633868f * s6;
     e.cb = cb;
+    e.s6 = s6;
     e.jh = ne * (float)(1.4142135623730951 * kPi / 3. / 9.) * (nu_c * sth) * (ff * ff);
-    const float inu = frcp(nu_p);
     const float x = (float)(kPlanck / (kElectronMass * kC * kC)) * nu_p * frcp(Te);
     const float cn = (float)kC * inu;
     e.kap = (float)(0.5 * kE2CH) * fexpm1(x) * (cn * cn) * inu;
     return e;
 }
 
-// d ln j / d ln X of the fit at fixed n_e and nu_s
-__host__ __device__ inline float dlnj_dlnX(float X, float cb)
+__host__ __device__ inline EmisF emission_f32(float B, float mu, float Te, float nu_p, float ne)
+{
+    return emission_f32(B, mu, Te, nu_p, ne, frcp(nu_p));
+}
+
+// d ln j / d ln X of the fit at fixed n_e and nu_s, from s6 = X^(1/6)
+__host__ __device__ inline float dlnj_dlnX(float s6)
 {
     const float c1112 = 1.8877486253633868f;
-    float sx = fsqrt(X), t = fsqrt(cb);
-    return 2.f * (0.5f * sx + c1112 / 6.f * t) * frcp(sx + c1112 * t) - cb * (1.f / 3.f);
+    const float cb = s6 * s6, sx = cb * s6;
+    return 2.f * (0.5f * sx + c1112 / 6.f * s6) * frcp(sx + c1112 * s6) - cb * (1.f / 3.f);
 }
 
 // Emissivity and absorptivity of the other emission models: j_nu = (e^2/c) jx
@@ -898,13 +909,19 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
 {
     const float lnth = 6.907755278982137f;   // log(1000.)
     const float dl = (float)(dlabs * P.Rg);
+    // For the thermal fit, g enters only where j != 0, hence nu_p > 0 and
+    // g = gdop, so its reciprocals are taken once for all frequencies (with
+    // j = 0 the factor only has to be finite: gdop <= 0 gives 0).
+    const float rg = frcp(F.gdop);
+    const float dlg2 = F.gdop > 0.f ? dl * frcp(F.gdop * F.gdop) : 0.f;
 #pragma unroll
     for (int f = 0; f < NF; f++) {
         float g = F.gdop;
         float nu_p = g * P.fnu[f];
         if (nu_p <= 0.f) {
             nu_p = 1.f;
-            g = frcp(P.fnu[f]);
+            if constexpr (EMIS != kThSynch)
+                g = frcp(P.fnu[f]);
         }
         if constexpr (EMIS != kThSynch) {
             static_assert(!GRAD, "the tangents are those of the thermal fit");
@@ -922,15 +939,15 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
                 kahan_add(acc.I[f], acc.cI[f], dI);
             continue;
         }
-        const EmisF e = emission_f32(F.B, F.mu, F.thetae, nu_p, F.ne);
+        const EmisF e = emission_f32(F.B, F.mu, F.thetae, nu_p, F.ne, rg * P.finu[f]);
         const float jx = e.jh * fexp(-e.cb);
         float dtau = e.jh != 0.f ? g * dl * jx * e.kap : (e.poison ? NAN : 0.f);
         kahan_add(acc.tau[f], acc.ctau[f], dtau);
         const float tau = acc.tau[f];
-        const float dI = (float)kE2C * jx * fexp(-tau) * (dl * frcp(g * g));
+        const float dI = (float)kE2C * jx * fexp(-tau) * dlg2;
         float dlnj0 = 0.f, dlnj1 = 0.f, dlna1 = 0.f;
         if (GRAD && e.jh != 0.f) {
-            float Dx = dlnj_dlnX(e.X, e.cb);
+            float Dx = dlnj_dlnX(e.s6);
             float dlnB = 0.5f * (F.B - 1e-20f) * frcp(F.B);
             float dlnT = F.dthetae * frcp(F.thetae);
             float x = (float)(kPlanck / (kElectronMass * kC * kC)) * nu_p * frcp(F.thetae);
@@ -1272,6 +1289,7 @@ inline const char *fill_params(const RtParams &p, const RtGrid &g, const double 
             return "frequencies must be positive";
         P.freq[f] = nu;
         P.fnu[f] = (float)nu;
+        P.finu[f] = (float)(1. / nu);
     }
     P.ncam = ncam;
     for (int i = 0; i < ncam; i++) {
