@@ -130,10 +130,17 @@ __host__ inline float fexp2(float x) { return exp2f(x); }
 __host__ inline float flog2(float x) { return log2f(x); }
 __host__ __device__ inline float fexp(float x) { return fexp2(1.4426950408889634f * x); }
 
+// Whether any lane of the wave holds c: a wave-uniform branch condition, so
+// that the branch is skipped rather than evaluated and selected away.
+__device__ inline bool wave_any(bool c) { return __ballot(c) != 0ull; }
+__host__ inline bool wave_any(bool c) { return c; }
+
 // e^x - 1 for the Planck factor: the Taylor series to x^6/6! for |x| <= 1/4
 // (truncation below x^6/7! < 5e-8 relative) and exp(x) - 1 beyond (the
-// single-instruction exp2's ~1 ulp, at most 5.4e-7 relative at |x| = 1/4);
-// 12 instructions instead of the library's 25.
+// single-instruction exp2's ~1 ulp, at most 5.4e-7 relative at |x| = 1/4).
+// The exp side is a branch, skipped by a wave none of whose lanes needs it:
+// x = h nu / (k T_e) stays below 1e-5 for radio to sub-mm frequencies at
+// every temperature the emissivity gates let through (Theta_e > 1e-4).
 __host__ __device__ inline float fexpm1(float x)
 {
     float p = 1.f / 720.f;
@@ -143,8 +150,13 @@ __host__ __device__ inline float fexpm1(float x)
     p = p * x + 0.5f;
     p = p * x + 1.f;
     const float t = x * p;
-    const float e = fexp(x) - 1.f;
-    return fabsf(x) <= 0.25f ? t : e;
+    const bool far = !(fabsf(x) <= 0.25f);
+    if (wave_any(far)) {
+        asm volatile("");   // keeps the exp inside the branch (not speculated)
+        const float e = fexp(x) - 1.f;
+        return far ? e : t;
+    }
+    return t;
 }
 
 // FP64 reciprocal for the geodesic step: the hardware estimate refined by two
@@ -710,50 +722,68 @@ __host__ __device__ inline bool fluid_sample(const Params &P, const float4 *__re
 // jh == 0 marks the reference's zero-emissivity gates.  When j is zero and the
 // reference's FP64 exp(x) overflows (x > 709.78, only for Theta_e below
 // ~1e-11) its absorptivity is 0/0 = NaN; `poison` reports that case.
-// X^(1/2) and X^(1/3) are powers of s6 = X^(1/6), one exp2 for all three
-// roots; inu = 1 / nu_p comes from the caller, which forms it as (1/g)(1/nu)
-// from a per-sample and a per-frequency reciprocal.
+// The fit is split into what one fluid sample fixes (EmisPre, once per
+// sample) and what each frequency adds (emission_nu).  X^(1/2) and X^(1/3)
+// are powers of s6 = X^(1/6), one exp2 for all three roots; inu = 1 / nu_p
+// comes from the caller, which forms it as (1/g)(1/nu) from a per-sample and
+// a per-frequency reciprocal.
 struct EmisF {
     float jh, cb, kap, X, s6;
     bool poison;
 };
 
-__host__ __device__ inline EmisF emission_f32(float B, float mu, float Te, float nu_p, float ne,
-                                              float inu)
+struct EmisPre {
+    float inus;   // 1 / nu_s
+    float jpre;   // n_e sqrt(2) pi / 27 nu_c sin
+    float xk;     // h / (m_e c^2 Theta_e): x = xk nu_p
+    float Te;
+    bool on;      // the sin and Theta_e gates pass
+};
+
+__host__ __device__ inline EmisPre emission_pre(float B, float mu, float Te, float ne)
 {
-    EmisF e;
+    EmisPre p;
     const float sth = fsqrt(1.f - mu * mu);
     const float nu_c = (float)(kElectronCharge / (2. * kPi * kElectronMass * kC)) * B;
     const float nu_s = (2.f / 9.f) * nu_c * Te * Te * sth;
-    const float X = nu_p * frcp(nu_s);
-    e.X = X;
+    p.inus = frcp(nu_s);
+    p.jpre = ne * (float)(1.4142135623730951 * kPi / 3. / 9.) * (nu_c * sth);
+    p.xk = (float)(kPlanck / (kElectronMass * kC * kC)) * frcp(Te);
+    p.Te = Te;
+    p.on = !(fabsf(sth) < 1.e-10f) && Te > 1e-4f;
+    return p;
+}
+
+__host__ __device__ inline EmisF emission_nu(const EmisPre &p, float nu_p, float inu)
+{
+    EmisF e;
+    e.X = nu_p * p.inus;
     e.poison = false;
-    if (nu_p == 1.f || fabsf(sth) < 1.e-10f || !(Te > 1e-4f)) {
+    if (nu_p == 1.f || !p.on) {
         e.jh = 0.f;
         e.cb = 0.f;
         e.kap = 0.f;
         e.s6 = 0.f;
-        if (!(Te > 1e-4f)) {
-            double T = (double)Te * kElectronMass * kC * kC / kBoltzmann;
+        if (!(p.Te > 1e-4f)) {
+            double T = (double)p.Te * kElectronMass * kC * kC / kBoltzmann;
             e.poison = kPlanck * (double)nu_p / (kBoltzmann * T) > 709.782712893384;
         }
         return e;
     }
-    const float s6 = fexp2(flog2(X) * (1.f / 6.f));
+    const float s6 = fexp2(flog2(e.X) * (1.f / 6.f));
     const float cb = s6 * s6;                                         // X^(1/3)
     const float ff = cb * s6 + 1.8877486253633868f * s6;
     e.cb = cb;
     e.s6 = s6;
-    e.jh = ne * (float)(1.4142135623730951 * kPi / 3. / 9.) * (nu_c * sth) * (ff * ff);
-    const float x = (float)(kPlanck / (kElectronMass * kC * kC)) * nu_p * frcp(Te);
+    e.jh = p.jpre * (ff * ff);
     const float cn = (float)kC * inu;
-    e.kap = (float)(0.5 * kE2CH) * fexpm1(x) * (cn * cn) * inu;
+    e.kap = (float)(0.5 * kE2CH) * fexpm1(p.xk * nu_p) * (cn * cn) * inu;
     return e;
 }
 
 __host__ __device__ inline EmisF emission_f32(float B, float mu, float Te, float nu_p, float ne)
 {
-    return emission_f32(B, mu, Te, nu_p, ne, frcp(nu_p));
+    return emission_nu(emission_pre(B, mu, Te, ne), nu_p, frcp(nu_p));
 }
 
 // d ln j / d ln X of the fit at fixed n_e and nu_s, from s6 = X^(1/6)
@@ -914,6 +944,7 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
     // j = 0 the factor only has to be finite: gdop <= 0 gives 0).
     const float rg = frcp(F.gdop);
     const float dlg2 = F.gdop > 0.f ? dl * frcp(F.gdop * F.gdop) : 0.f;
+    const EmisPre pre = emission_pre(F.B, F.mu, F.thetae, F.ne);
 #pragma unroll
     for (int f = 0; f < NF; f++) {
         float g = F.gdop;
@@ -939,7 +970,7 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
                 kahan_add(acc.I[f], acc.cI[f], dI);
             continue;
         }
-        const EmisF e = emission_f32(F.B, F.mu, F.thetae, nu_p, F.ne, rg * P.finu[f]);
+        const EmisF e = emission_nu(pre, nu_p, rg * P.finu[f]);
         const float jx = e.jh * fexp(-e.cb);
         float dtau = e.jh != 0.f ? g * dl * jx * e.kap : (e.poison ? NAN : 0.f);
         kahan_add(acc.tau[f], acc.ctau[f], dtau);
@@ -950,7 +981,7 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
             float Dx = dlnj_dlnX(e.s6);
             float dlnB = 0.5f * (F.B - 1e-20f) * frcp(F.B);
             float dlnT = F.dthetae * frcp(F.thetae);
-            float x = (float)(kPlanck / (kElectronMass * kC * kC)) * nu_p * frcp(F.thetae);
+            float x = pre.xk * nu_p;
             float dlnBnu = x * frcp(-fexpm1(-x));
             dlnj0 = 1.f + dlnB * (1.f - Dx);
             dlnj1 = -2.f * Dx * dlnT;
