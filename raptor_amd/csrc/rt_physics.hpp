@@ -102,7 +102,8 @@ struct Params {
     double freq[kMaxFreq];
     // FP32 copies for the transfer
     float fnu[kMaxFreq];
-    float finu[kMaxFreq];        // 1 / freq, rounded once from FP64
+    float flnu6[kMaxFreq];       // (log2(freq) - 37) / 6
+    float fkap[kMaxFreq];        // (e^2 / (c h)) c^2 / (2 freq^3)
     float fNe_unit, fB_unit, fiB_unit, fR_high, fR_low, fa, fpi_h;   // fpi_h = (1 - h) pi
     int emission;                // kThSynch, kThSynchAv or kKappa
     KappaConst kc;
@@ -724,11 +725,14 @@ __host__ __device__ inline bool fluid_sample(const Params &P, const float4 *__re
 // ~1e-11) its absorptivity is 0/0 = NaN; `poison` reports that case.
 // The fit is split into what one fluid sample fixes (EmisPre, once per
 // sample) and what each frequency adds (emission_nu).  X^(1/2) and X^(1/3)
-// are powers of s6 = X^(1/6), one exp2 for all three roots; inu = 1 / nu_p
-// comes from the caller, which forms it as (1/g)(1/nu) from a per-sample and
-// a per-frequency reciprocal.
+// are powers of s6 = X^(1/6), one exp2 for all three roots.  The caller
+// passes s6 and kf = (e^2 / (c h)) c^2 / (2 nu_p^3): with nu_p = g nu, the
+// transfer forms them from per-sample and per-frequency factors,
+// log2 s6 = (log2(g 2^37 / nu_s) + log2(nu / 2^37)) / 6 and kf = g^-3 kf(nu);
+// both logs stay of the size of log2 X near 2^37 Hz = 137 GHz, so their sum
+// loses no more than log2 X itself would.
 struct EmisF {
-    float jh, cb, kap, X, s6;
+    float jh, cb, kap, s6;
     bool poison;
 };
 
@@ -754,10 +758,9 @@ __host__ __device__ inline EmisPre emission_pre(float B, float mu, float Te, flo
     return p;
 }
 
-__host__ __device__ inline EmisF emission_nu(const EmisPre &p, float nu_p, float inu)
+__host__ __device__ inline EmisF emission_nu(const EmisPre &p, float nu_p, float s6, float kf)
 {
     EmisF e;
-    e.X = nu_p * p.inus;
     e.poison = false;
     if (nu_p == 1.f || !p.on) {
         e.jh = 0.f;
@@ -770,20 +773,21 @@ __host__ __device__ inline EmisF emission_nu(const EmisPre &p, float nu_p, float
         }
         return e;
     }
-    const float s6 = fexp2(flog2(e.X) * (1.f / 6.f));
     const float cb = s6 * s6;                                         // X^(1/3)
     const float ff = cb * s6 + 1.8877486253633868f * s6;
     e.cb = cb;
     e.s6 = s6;
     e.jh = p.jpre * (ff * ff);
-    const float cn = (float)kC * inu;
-    e.kap = (float)(0.5 * kE2CH) * fexpm1(p.xk * nu_p) * (cn * cn) * inu;
+    e.kap = fexpm1(p.xk * nu_p) * kf;
     return e;
 }
 
 __host__ __device__ inline EmisF emission_f32(float B, float mu, float Te, float nu_p, float ne)
 {
-    return emission_nu(emission_pre(B, mu, Te, ne), nu_p, frcp(nu_p));
+    const EmisPre p = emission_pre(B, mu, Te, ne);
+    const float inu = frcp(nu_p), cn = (float)kC * inu;
+    return emission_nu(p, nu_p, fexp2(flog2(nu_p * p.inus) * (1.f / 6.f)),
+                       (float)(0.5 * kE2CH) * (cn * cn) * inu);
 }
 
 // d ln j / d ln X of the fit at fixed n_e and nu_s, from s6 = X^(1/6)
@@ -945,6 +949,13 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
     const float rg = frcp(F.gdop);
     const float dlg2 = F.gdop > 0.f ? dl * frcp(F.gdop * F.gdop) : 0.f;
     const EmisPre pre = emission_pre(F.B, F.mu, F.thetae, F.ne);
+    // log2 X at 2^37 Hz (the power of two scales exactly), to which each
+    // frequency adds log2(nu / 2^37) in an explicit FMA: a separate product
+    // would be contracted into the sum or not depending on how many
+    // frequencies share it, and launches of different frequency counts must
+    // give identical bits
+    const float lgs = flog2(F.gdop * pre.inus * 0x1p37f);
+    const float rg3 = rg * rg * rg;
 #pragma unroll
     for (int f = 0; f < NF; f++) {
         float g = F.gdop;
@@ -970,7 +981,8 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
                 kahan_add(acc.I[f], acc.cI[f], dI);
             continue;
         }
-        const EmisF e = emission_nu(pre, nu_p, rg * P.finu[f]);
+        const EmisF e =
+            emission_nu(pre, nu_p, fexp2(fmaf(lgs, 1.f / 6.f, P.flnu6[f])), rg3 * P.fkap[f]);
         const float jx = e.jh * fexp(-e.cb);
         float dtau = e.jh != 0.f ? g * dl * jx * e.kap : (e.poison ? NAN : 0.f);
         kahan_add(acc.tau[f], acc.ctau[f], dtau);
@@ -1320,7 +1332,8 @@ inline const char *fill_params(const RtParams &p, const RtGrid &g, const double 
             return "frequencies must be positive";
         P.freq[f] = nu;
         P.fnu[f] = (float)nu;
-        P.finu[f] = (float)(1. / nu);
+        P.flnu6[f] = (float)((std::log2(nu) - 37.) / 6.);
+        P.fkap[f] = (float)(0.5 * kE2CH * kC * kC / (nu * nu * nu));
     }
     P.ncam = ncam;
     for (int i = 0; i < ncam; i++) {
