@@ -252,7 +252,11 @@ __device__ inline void start_ray(const Params &P, const TraceArgs &A, int ray, i
 // so every wave drains.
 // One frequency (except the 3D gradient pass) fits in 168 VGPRs without
 // spilling, so those kernels are also built for 3 waves per SIMD, which hide
-// more of the FP64 dependency latency: 6% on the 1024^2 image.  A launch
+// more of the FP64 dependency latency: 6% on the 1024^2 image.  Images of up
+// to 8 frequencies of a 2D dump without gradients are built so too: at 8
+// frequencies the bound spills a few per-ray counters, reloaded once a step,
+// and the third wave still gains 10% (6 inclinations x 8 frequencies at
+// 512^2).  A launch
 // with few rays per lane is instead bound by its longest rays, which a third
 // wave sharing the SIMD slows down (512^2: 10% slower), so the 3-wave build
 // is used from kThreeWaveRays rays per launch.  The other kernels are left to
@@ -264,7 +268,7 @@ static constexpr int kLongRay = 800;   // steps: about the 99.5th percentile at 
 template <int NF, bool GRAD, bool DIM3, int INT>
 constexpr bool three_waves()
 {
-    return INT == kRK2 && NF == 1 && !(GRAD && DIM3);
+    return INT == kRK2 && (NF == 1 ? !(GRAD && DIM3) : (NF <= 8 && !GRAD && !DIM3));
 }
 
 template <int NF, bool GRAD, bool DIM3, int INT, bool RESUME, int EMIS>

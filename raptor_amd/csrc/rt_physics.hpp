@@ -902,20 +902,22 @@ __host__ __device__ inline EmisX emission_thsynchav(float B, float Te, float nu_
     return e;
 }
 
-// Per-ray accumulators for NF frequencies: optical depth and I_nu nu^3
-// (FP32 with Kahan compensation), and the tangents (FP64: they difference
-// nearly equal terms, d ln j - d tau).
+// Per-ray accumulators for NF frequencies: optical depth (FP32 with Kahan
+// compensation: the ln 1000 gate compares it), I_nu nu^3 (a plain FP32 sum of
+// positive terms, relative rounding error about sqrt(n) 2^-24 for n samples,
+// at most n 2^-24), and the tangents (FP64: they difference nearly equal
+// terms, d ln j - d tau).
 template <int NF, bool GRAD>
 struct Accum {
     float tau[NF], ctau[NF];
-    float I[NF], cI[NF];
+    float I[NF];
     double dtau[GRAD ? NF : 1][2];
     double dI[GRAD ? NF : 1][2];
     __host__ __device__ void clear()
     {
         for (int f = 0; f < NF; f++) {
             tau[f] = ctau[f] = 0.f;
-            I[f] = cI[f] = 0.f;
+            I[f] = 0.f;
             if (GRAD) {
                 dtau[f][0] = dtau[f][1] = 0.;
                 dI[f][0] = dI[f][1] = 0.;
@@ -978,7 +980,7 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
             const float tau = acc.tau[f];
             const float dI = (float)kE2C * e.jx * fexp(-tau) * (dl * frcp(g * g));
             if (tau < lnth && !(tau < 0.f))
-                kahan_add(acc.I[f], acc.cI[f], dI);
+                acc.I[f] += dI;
             continue;
         }
         const EmisF e =
@@ -1004,7 +1006,7 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
             acc.dtau[f][1] += (double)dtau * dlna1;
         }
         if (tau < lnth && !(tau < 0.f)) {
-            kahan_add(acc.I[f], acc.cI[f], dI);
+            acc.I[f] += dI;
             if (GRAD && dI != 0.f) {
                 acc.dI[f][0] += (double)dI * (dlnj0 - acc.dtau[f][0]);
                 acc.dI[f][1] += (double)dI * (dlnj1 - acc.dtau[f][1]);

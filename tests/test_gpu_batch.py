@@ -170,13 +170,22 @@ def test_centre_order_covers_every_pixel_once(tracer):
             tracer.free(p)
 
 
-def test_three_wave_build_equals_two_wave_build(tracer):
+@pytest.mark.parametrize("nf", [1, 3, 8])
+def test_three_wave_build_equals_two_wave_build(tracer, nf):
     """Launches of at least 2^19 rays run the kernel built for 3 waves per
-    SIMD, smaller ones the 2-wave build; the same rays give the same bits."""
+    SIMD (up to 8 frequencies), smaller ones the 2-wave build; the same rays
+    give the same bits, also when the approach kernel hands them over."""
+    freqs = [FREQ] if nf == 1 else C2_FREQ[:nf]
     _params(tracer, 512)
-    big, _ = tracer.render([INCL, 30.0], [FREQ])      # 524288 rays: 3-wave build
-    small, _ = tracer.render([INCL], [FREQ])          # 262144 rays: 2-wave build
+    small, _ = tracer.render([INCL], freqs)          # 262144 rays: 2-wave build
+    big, _ = tracer.render([INCL, 30.0], freqs)      # 524288 rays: 3-wave build
     assert np.array_equal(big[0], small[0])
+    tracer.set_launch_config(approach=100)
+    try:
+        split, _ = tracer.render([INCL, 30.0], freqs)
+    finally:
+        tracer.set_launch_config()
+    assert np.array_equal(split, big)
     assert big[0].max() > 0
 
 
