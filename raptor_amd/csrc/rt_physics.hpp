@@ -87,6 +87,7 @@ struct Params {
     double startx1, startx2, startx3;
     double dx1, dx2, dx3;
     double idx1, idx2, idx3;     // 1 / dx
+    double period3, iperiod3;    // n3 dx3 and its reciprocal (the X3 period)
     double stopx1, stopx2;
     double a, hslope;
     // camera and integrator
@@ -547,16 +548,25 @@ __host__ __device__ inline void sample_prims(const Params &P, const float4 *__re
         v[6] = RT_BI(p00b.z, p01b.z, p10b.z, p11b.z);
         v[7] = RT_BI(p00b.w, p01b.w, p10b.w, p11b.w);
     } else {
-        // periodic in X3 = phi; cell centres at startx3 + (k + 1/2) dx3
-        double period = P.n3 * P.dx3;
-        double ph = fmod(y[3] - P.startx3, period);
+        // periodic in X3 = phi; cell centres at startx3 + (k + 1/2) dx3.  The
+        // remainder x - P floor(x / P) (one FMA) is fmod's up to a rounding
+        // when x / P rounds across an integer, which the fix-up brings back
+        // into [0, P); the weights are continuous there, so either side gives
+        // the same sample to rounding.
+        const double x3 = y[3] - P.startx3;
+        double ph = fma(-floor(x3 * P.iperiod3), P.period3, x3);
         if (ph < 0.)
-            ph += period;
+            ph += P.period3;
+        else if (ph >= P.period3)
+            ph -= P.period3;
         double t = ph * P.idx3 - 0.5;
         double fk = floor(t);
         const float d3 = (float)(t - fk);
+        // t lies in [-1/2, n3 - 1/2), so k in [-1, n3 - 1] (clamped: a NaN
+        // position converts to an arbitrary integer on some targets)
         int k = (int)fk;
-        int k0 = (k + P.n3) % P.n3, k1 = (k + 1) % P.n3;
+        k = k < -1 ? -1 : (k > P.n3 - 1 ? P.n3 - 1 : k);
+        const int k0 = k < 0 ? P.n3 - 1 : k, k1 = k + 1 == P.n3 ? 0 : k + 1;
         const size_t n3 = P.n3, n2 = P.n2;
         size_t b00 = ((size_t)i * n2 + j) * n3, b01 = b00 + n3;
         size_t b10 = b00 + n2 * n3, b11 = b10 + n3;
@@ -1298,6 +1308,8 @@ inline const char *fill_params(const RtParams &p, const RtGrid &g, const double 
     P.idx1 = 1. / g.dx1;
     P.idx2 = 1. / g.dx2;
     P.idx3 = g.n3 > 1 ? 1. / g.dx3 : 0.;
+    P.period3 = g.n3 * g.dx3;
+    P.iperiod3 = g.n3 > 1 ? 1. / P.period3 : 0.;
     P.stopx1 = g.startx1 + g.n1 * g.dx1;
     P.stopx2 = g.startx2 + g.n2 * g.dx2;
     P.a = g.a;

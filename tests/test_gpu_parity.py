@@ -428,7 +428,17 @@ def test_probe_emission(tracer):
     q = np.column_stack([10 ** rng.uniform(-1, 3, n), rng.uniform(-1, 1, n),
                          10 ** rng.uniform(-5, 2, n), 10 ** rng.uniform(9, 13, n),
                          10 ** rng.uniform(2, 8, n)])
+    # every other row: fields of 1e15-1e17 G just above the Theta_e gate at
+    # 3e15-3e16 Hz, where h nu / (m c^2 Theta_e) spans 0.08-2.4 while
+    # X^(1/3) stays below ~20, so the Planck factor's exp branch (|x| > 1/4)
+    # runs in waves whose other lanes take the series
+    far = np.column_stack([10 ** rng.uniform(15, 17, n // 2), rng.uniform(-0.9, 0.9, n // 2),
+                           10 ** rng.uniform(-3.99, -3.5, n // 2),
+                           10 ** rng.uniform(15.5, 16.5, n // 2), 10 ** rng.uniform(2, 8, n // 2)])
+    q[1::2] = far
     out = tracer.probe(rt.PROBE_EMISSION, q)
+    xs = 6.6260693e-27 * q[:, 3] / (q[:, 2] * 9.1093826e-28 * 2.99792458e10 ** 2)
+    assert (xs > 0.25).sum() > 20 and (xs[1::2] < 0.25).sum() > 20
     lib = O.lib()
     for i in range(n):
         j = lib.or_j_thsynch(q[i, 0], q[i, 1], q[i, 2], q[i, 3], q[i, 4])
@@ -446,6 +456,7 @@ def test_probe_emission(tracer):
             assert out[i, 1] == pytest.approx(a, rel=tol + 4e-16 / x, abs=1e-250)
         else:
             assert np.isnan(out[i, 1]) == np.isnan(a)
+    assert np.count_nonzero(out[1::2, 1][xs[1::2] > 0.25]) > 20
 
 
 def test_fluid_params_hook(tracer, dump128):
