@@ -450,13 +450,43 @@ def test_probe_emission(tracer):
         tol = 2e-5 + 1e-6 * np.cbrt(X)
         assert out[i, 0] == pytest.approx(j, rel=tol, abs=1e-300)
         # the oracle's Planck factor exp(x) - 1 (j_nu.c:190) loses ~1e-16/x;
-        # the kernel uses expm1
+        # the kernel's (a series below |x| = 1/4) does not
         x = 6.6260693e-27 * q[i, 3] / (q[i, 2] * 9.1093826e-28 * 2.99792458e10 ** 2)
         if np.isfinite(a):
             assert out[i, 1] == pytest.approx(a, rel=tol + 4e-16 / x, abs=1e-250)
         else:
             assert np.isnan(out[i, 1]) == np.isnan(a)
     assert np.count_nonzero(out[1::2, 1][xs[1::2] > 0.25]) > 20
+
+
+def test_fluid_params_hook_3d_periodic(tracer, dump3d_small):
+    """The 3D sampler's periodic phi: positions at, just below and just above
+    multiples of the period, negative and many periods out, against the
+    oracle's fmod-based index."""
+    P = _setup(tracer, dump3d_small, 8)
+    g = dump3d_small.grid
+    period = g.n3 * g.dx3
+    rng = np.random.default_rng(12)
+    offs = [0.0, 1e-15 * period, -1e-15 * period, 0.5 * g.dx3, -0.5 * g.dx3, g.dx3,
+            period - 1e-15 * period, 0.25 * period]
+    X3 = [g.startx3 + m * period + o for m in (-3, -1, 0, 1, 2, 7) for o in offs]
+    X3 += list(g.startx3 + rng.uniform(-20, 20, 52) * period)
+    n = len(X3)
+    X = np.zeros((n, 4))
+    X[:, 1] = rng.uniform(g.startx1 + 0.3 * (g.stopx1 - g.startx1), g.stopx1, n)
+    X[:, 2] = rng.uniform(g.startx2 + 0.2, g.stopx2 - 0.2, n)
+    X[:, 3] = X3
+    out = tracer.fluid_params(X)
+    checked = 0
+    for i in range(n):
+        ref = O.fluid_at(P, dump3d_small.prims, X[i])
+        assert bool(out[i, 11]) == ref["ok"]
+        if ref["ok"]:
+            assert out[i, 0] == pytest.approx(ref["ne"], rel=1e-6), X[i]
+            assert out[i, 1] == pytest.approx(ref["thetae"], rel=1e-5), X[i]
+            assert out[i, 2] == pytest.approx(ref["B"], rel=1e-4), X[i]
+            checked += 1
+    assert checked > n // 2
 
 
 def test_fluid_params_hook(tracer, dump128):
