@@ -197,7 +197,7 @@ def test_host_build_emission_models_unit(model):
         L.hc_emission(model, *row, ctypes.c_void_p(out.ctypes.data))
         j, a = O.emission(model, *row)
         # Kirchhoff's absorptivity: the oracle's Planck factor exp(x) - 1
-        # (j_nu.c:190) loses ~1e-16/x, the kernel uses expm1
+        # (j_nu.c:190) loses ~1e-16/x, the kernel's e^x - 1 does not
         x = 6.6260693e-27 * row[3] / (row[2] * 9.1093826e-28 * 2.99792458e10 ** 2)
         tola = 2e-4 + (4e-16 / x if model == O.THSYNCHAV else 0.0)
         for got, ref, tol in ((out[0], j, 2e-4), (out[1], a, tola)):
