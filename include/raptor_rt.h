@@ -119,7 +119,7 @@ int rt_load_prims(rt_ctx *ctx, const rt_grid *grid, const float *h_prims);
 /* ---- the pixel loop ----------------------------------------------------- */
 
 /* calculate_image for n_cam inclinations (degrees) and n_freq frequencies
- * (Hz) in one launch.  h_image: [n_cam][W*H][n_freq] doubles; h_spectrum
+ * (Hz, in (0, 1e17]: the FP32 transfer's range) in one launch.  h_image: [n_cam][W*H][n_freq] doubles; h_spectrum
  * (may be NULL): [n_cam][n_freq].  Synchronous. */
 int rt_render(rt_ctx *ctx, const double *inclinations, int n_cam,
               const double *freqs, int n_freq, double *h_image, double *h_spectrum);

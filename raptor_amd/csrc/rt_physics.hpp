@@ -1344,6 +1344,10 @@ inline const char *fill_params(const RtParams &p, const RtGrid &g, const double 
         double nu = freqs[std::min(f, nf - 1)];
         if (!(nu > 0.))
             return "frequencies must be positive";
+        // above ~1e18 Hz the FP32 Planck factor overflows (h nu > 88 k T_e at
+        // the Theta_e gate) and nu^-3 factors leave FP32's normal range
+        if (nu > 1e17)
+            return "frequencies above 1e17 Hz are outside the FP32 transfer's range";
         P.freq[f] = nu;
         P.fnu[f] = (float)nu;
         P.flnu6[f] = (float)((std::log2(nu) - 37.) / 6.);

@@ -527,5 +527,10 @@ def test_error_codes(dump128):
             tr.render([60.0], np.full(17, FREQ))
         with pytest.raises(rt.RtError, match="inclination"):
             tr.render([0.0], [FREQ])
+        with pytest.raises(rt.RtError, match="positive"):
+            tr.render([60.0], [FREQ, 0.0])
+        with pytest.raises(rt.RtError, match="1e17 Hz"):
+            tr.render([60.0], [FREQ, 2e17])
+        tr.render([60.0], [1e17])
         with pytest.raises(rt.RtError):
             tr.set_params(width=0, height=4, M_unit=M_UNIT, R_high=R_HIGH)
