@@ -141,8 +141,9 @@ __host__ inline bool wave_any(bool c) { return c; }
 // (truncation below x^6/7! < 5e-8 relative) and exp(x) - 1 beyond (the
 // single-instruction exp2's ~1 ulp, at most 5.4e-7 relative at |x| = 1/4).
 // The exp side is a branch, skipped by a wave none of whose lanes needs it:
-// x = h nu / (k T_e) stays below 1e-5 for radio to sub-mm frequencies at
-// every temperature the emissivity gates let through (Theta_e > 1e-4).
+// x = h nu / (k T_e) stays below 1e-4 for radio to sub-mm frequencies
+// (up to 1e12 Hz) at every temperature the emissivity gates let through
+// (Theta_e > 1e-4).
 __host__ __device__ inline float fexpm1(float x)
 {
     float p = 1.f / 720.f;
