@@ -529,6 +529,10 @@ def test_error_codes(dump128):
             tr.render([0.0], [FREQ])
         with pytest.raises(rt.RtError, match="positive"):
             tr.render([60.0], [FREQ, 0.0])
+        with pytest.raises(rt.RtError, match="n_cam"):
+            tr.render([], [FREQ])
+        with pytest.raises(rt.RtError, match="n_freq"):
+            tr.render([60.0], [])
         with pytest.raises(rt.RtError, match="1e17 Hz"):
             tr.render([60.0], [FREQ, 2e17])
         tr.render([60.0], [1e17])
