@@ -151,11 +151,16 @@ def main():
     split = launch["approach"] > 0 or (launch["approach"] == 0 and npix >= 1 << 20)
     kernels = ("approach_kernel<0> + trace_kernel_3w<1,false,false,0,true,0>" if split
                else "trace_kernel_3w<1,false,false,0,false,0>")
-    traffic = None
+    traffic = issue = None
     tfile = os.path.join(ROOT, "profiles", "trace_kernel_traffic.json")
-    if W == 1024 and os.path.exists(tfile):
+    if W == 1024 and split and os.path.exists(tfile):
         with open(tfile) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
+            prof = json.load(fh)
+        traffic = prof.get("bytes_per_launch")
+        insts = prof.get("valu_wave_insts_per_launch")
+        if insts:
+            # a wave64 VALU instruction occupies its SIMD for 4 cycles
+            issue = insts * 4 / (kern_ms * 1e-3 * 1024 * 2.4e9)
 
     result = {
         "metric": METRIC,
@@ -188,6 +193,10 @@ def main():
             "kernel_note": "the render's kernels, timed together with HIP events on the "
                            "library stream; the roofline covers their sum",
             "kernel_ms": kern_ms,
+            "valu_issue_frac": issue,
+            "valu_issue_note": "wave64 VALU instructions per render (rocprofv3 SQ_INSTS_VALU, "
+                               "profiles/trace_kernel_traffic.json) x 4 cycles over 1024 SIMDs "
+                               "x 2.4 GHz x kernel_ms: the share of issue slots used",
             "alg_flops_per_launch": alg_flops,
             "alg_flops_fp64": f64,
             "alg_flops_fp32": f32,
