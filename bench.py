@@ -49,6 +49,29 @@ def host_cores():
     return max(1, n)
 
 
+def _parity(g, ora, gate):
+    """GPU pixels g against the oracle's ora (same shape); gate: the oracle's
+    distance of tau from the ln 1000 gate at its closest sample."""
+    floor = 1e-6 * np.max(np.abs(ora))
+    r = np.abs(g - ora) / np.maximum(np.abs(ora), floor)
+    near = gate < 1e-4
+    return {
+        "pixels": int(g.shape[0]), "flux_rel": float(g.sum() / ora.sum() - 1.0),
+        "max_pix_rel": float(r.max()), "n_pix_over_1e-3": int((r > 1e-3).sum()),
+        "floor": "1e-6*max(I)",
+        "gate_adjacent": int(near.sum()),
+        "max_pix_rel_off_gate": float(r[~near].max()) if (~near).any() else 0.0,
+        "gate_note": "gate_adjacent: pixels whose FP64 tau passes within 1e-4 of the "
+                     "ln 1000 gate at a sample, where the FP32 tau may add or drop one "
+                     "exp(-6.9)-weighted sample",
+    }
+
+
+def _sample(npix, n):
+    stride = max(1, npix // n)
+    return np.arange(0, npix, stride, dtype=np.int32)[:n], stride
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,20 +265,7 @@ def main():
                       "schedule, %.1f s wall" % (npix, c1 - c0),
         }
         result["speedup_vs_cpu"] = value / cpu_rate
-        g = img[pix]
-        floor = 1e-6 * np.max(np.abs(ora))
-        r = np.abs(g - ora) / np.maximum(np.abs(ora), floor)
-        near = gate < 1e-4
-        result["parity"] = {
-            "pixels": int(n), "flux_rel": float(g.sum() / ora.sum() - 1.0),
-            "max_pix_rel": float(r.max()), "n_pix_over_1e-3": int((r > 1e-3).sum()),
-            "floor": "1e-6*max(I)",
-            "gate_adjacent": int(near.sum()),
-            "max_pix_rel_off_gate": float(r[~near].max()),
-            "gate_note": "gate_adjacent: pixels whose FP64 tau passes within 1e-4 of the "
-                         "ln 1000 gate at a sample, where the FP32 tau may add or drop one "
-                         "exp(-6.9)-weighted sample",
-        }
+        result["parity"] = _parity(img[pix], ora, gate)
 
     tr.free(d_img)
     tr.free(d_dimg)
@@ -346,6 +356,34 @@ def bench_c2(args):
     tr.free(d_s)
     tr.close()
     rays = len(incl) * npix
+    cpu = parity = None
+    if args.cpu_sample > 0:
+        # the FP64 oracle on every 64th pixel of each inclination, all 8
+        # frequencies: the CPU baseline, and parity for all 48 images
+        from oracle import oracle as O
+        pix, stride = _sample(npix, min(npix // 64, args.cpu_sample))
+        cores = host_cores()
+        oras, gates = [], []
+        c0 = time.perf_counter()
+        for inc in incl:
+            P = O.params(dump.grid, width=W, height=W, inclination=inc,
+                         M_unit=WORKLOAD["M_unit"], R_high=WORKLOAD["R_high"],
+                         R_low=WORKLOAD["R_low"])
+            o, gt = O.render(P, dump.prims, freqs, pixels=pix, gate=True, nthreads=cores)
+            oras.append(o)
+            gates.append(gt)
+        c1 = time.perf_counter()
+        n = len(incl) * pix.size
+        cpu = {"value": n / (c1 - c0), "unit": "pixel-geodesics/s", "cores": cores,
+               "kind": "port",
+               "sample": "every %d-th pixel of each of the %d inclinations (%d rays, %d "
+                         "frequencies each), FP64 oracle, OpenMP dynamic schedule, %.1f s "
+                         "wall" % (stride, len(incl), n, len(freqs), c1 - c0)}
+        parity = _parity(b[:, pix, :].reshape(-1), np.stack(oras).reshape(-1),
+                         np.stack(gates).reshape(-1))
+        parity["images"] = len(incl) * len(freqs)
+        parity["max_image_flux_rel"] = float(np.max(np.abs(
+            b[:, pix, :].sum(axis=1) / np.stack(oras).sum(axis=1) - 1.0)))
     print(json.dumps({
         "metric": METRIC, "workload": "c2", "unit": "pixel-geodesics/s",
         "value": rays / (wb * 1e-3), "ms_per_step": wb, "steps": args.steps,
@@ -357,6 +395,8 @@ def bench_c2(args):
                             "wall_ms": ws,
                             "images_per_s": len(incl) * len(freqs) / (ws * 1e-3)},
         "speedup_batched": ws / wb, "batched_equals_per_image": same,
+        "cpu_baseline": cpu, "parity": parity,
+        "speedup_vs_cpu": rays / (wb * 1e-3) / cpu["value"] if cpu else None,
         "autotune_table": [[c, round(m, 3)] for c, m in tune["table"]],
     }), flush=True)
 
@@ -389,14 +429,37 @@ def bench_3d(args):
                           prims_MB=dump.prims.size * 4 / 1e6),
            "image": {"kernel_ms": ki, "wall_ms": wi, "images_per_s": 1e3 / wi,
                      "roofline": _roofline(tr, incl, freq, ki)}}
+    img = tr.d2h(d_img, (npix,), np.float64)
+    if args.cpu_sample > 0:
+        # the FP64 oracle on a pixel sample: the CPU baseline and parity
+        from oracle import oracle as O
+        pix, stride = _sample(npix, min(npix // 64, args.cpu_sample))
+        P = O.params(dump.grid, width=W, height=W, inclination=incl[0],
+                     M_unit=WORKLOAD["M_unit"], R_high=WORKLOAD["R_high"],
+                     R_low=WORKLOAD["R_low"])
+        cores = host_cores()
+        c0 = time.perf_counter()
+        ora0, gate0 = O.render(P, dump.prims, freq, pixels=pix, gate=True, nthreads=cores)
+        c1 = time.perf_counter()
+        out["cpu_baseline"] = {
+            "value": pix.size / (c1 - c0), "unit": "pixel-geodesics/s", "cores": cores,
+            "kind": "port",
+            "sample": "every %d-th pixel (%d rays), FP64 oracle, OpenMP dynamic schedule, "
+                      "%.1f s wall" % (stride, pix.size, c1 - c0)}
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+        out["parity"] = _parity(img[pix], ora0[:, 0], gate0[:, 0])
     if args.workload == "c4":
         tr.render_device(incl, freq, d_img, d_dimg)
         tr.synchronize()
         kg, wg = _timed(tr, lambda: tr.render_device(incl, freq, d_img, d_dimg), args.steps)
         an = tr.d2h(d_dimg, (npix, 2), np.float64)
-        # central differences (step 1e-3 and 5e-4) of the FP64 oracle on a
-        # pixel sample; GPU-image differences would be FP32-noise limited
-        from oracle import oracle as O
+        out["grad"] = {"kernel_ms": kg, "wall_ms": wg, "image_plus_grad_per_s": 1e3 / wg,
+                       "ratio_vs_image": wi / wg,
+                       "roofline": _roofline(tr, incl, freq, kg, grad=True)}
+    if args.workload == "c4" and args.cpu_sample > 0:
+        # the CPU leg's check of the gradient: central differences (step 1e-3
+        # and 5e-4) of the FP64 oracle on a pixel sample; GPU-image
+        # differences would be FP32-noise limited
         M0, R0 = WORKLOAD["M_unit"], WORKLOAD["R_high"]
         pix = np.arange(0, npix, max(1, npix // 2048), dtype=np.int32)[:2048]
 
@@ -422,12 +485,10 @@ def bench_3d(args):
                            "max_rel_smooth": float(err[smooth].max()),
                            "frac_within_1e-3": float((err[smooth] < 1e-3).mean()),
                            "sum_rel": float(fds[0][smooth].sum() / a[smooth].sum() - 1)}
-        out["grad"] = {"kernel_ms": kg, "wall_ms": wg, "image_plus_grad_per_s": 1e3 / wg,
-                       "ratio_vs_image": wi / wg, "fd_check": stats,
-                       "roofline": _roofline(tr, incl, freq, kg, grad=True),
-                       "fd_note": "GPU tangents vs central differences (step 1e-3) of the "
+        out["grad"]["fd_check"] = stats
+        out["grad"]["fd_note"] = ("GPU tangents vs central differences (step 1e-3) of the "
                                   "FP64 oracle on 2048 sampled pixels; pixels whose quotient "
-                                  "changes between steps 1e-3 and 5e-4 (gate flips) excluded"}
+                                  "changes between steps 1e-3 and 5e-4 (gate flips) excluded")
     tr.free(d_img)
     tr.free(d_dimg)
     tr.close()
