@@ -176,11 +176,15 @@ __device__ inline void load_state(const RayState &S, int i, Ray<NF, GRAD> &R)
     R.pt.c = f[10 * 64];
     R.pt.s2 = f[11 * 64];
     R.pt.c2 = f[12 * 64];
-    R.steps = k[0];
-    R.turns = k[64];
+    // the fields in ray_init's order: the two set-ups are alternatives of one
+    // branch, and the compiler pairs their stores in order when it merges
+    // them, so a different order would pair different fields and leave the
+    // counters in scratch memory behind a selected address
     const int fl = k[192];
     R.thetadot_sign = (fl & 3) - 1;
     R.beta_sign = ((fl >> 2) & 3) - 1;
+    R.steps = k[0];
+    R.turns = k[64];
     R.samples = 0;
     R.terminate = false;
     R.acc.clear();
