@@ -524,11 +524,26 @@ struct rt_ctx {
     std::vector<void *> capture_pins;   // tables and buffers used by the open capture
     double *d_image = nullptr, *d_dimage = nullptr;
     size_t image_cap = 0, dimage_cap = 0;
+    // the kappa absorptivity's hypergeometric table (kappa_table), made once
+    float2 *d_ktab = nullptr;
+    float ktab_u0 = 0.f, ktab_idu = 1.f;
+    int ktab_n = 0;
     double last_ms = 0.;
     rt_launch_config cfg{64, 0, kDefaultRefill, kDefaultTile, RT_ORDER_CENTRE, 0};
     bool capturing = false;   // between rt_capture_begin and rt_capture_end
     char err[512] = {0};
 };
+
+// The context's kappa table in the kappa fit's constants.
+static KappaConst kappa_with_table(const rt_ctx *c)
+{
+    KappaConst K = kappa_const();
+    K.tab = c->d_ktab;
+    K.tu0 = c->ktab_u0;
+    K.tidu = c->ktab_idu;
+    K.tn = c->ktab_n;
+    return K;
+}
 
 struct rt_graph {
     hipGraph_t graph = nullptr;
@@ -596,6 +611,7 @@ static int make_params(rt_ctx *c, const double *incl, int ncam, const double *fr
     const char *msg = fill_params(p, g, incl, ncam, freqs, nf, P);
     if (msg)
         return fail(c, RT_E_INVALID, "%s", msg);
+    P.kc = kappa_with_table(c);
     return RT_OK;
 }
 
@@ -981,8 +997,19 @@ rt_ctx *rt_create(int device)
         return nullptr;
     }
     c->num_cu = prop.multiProcessorCount;
+    std::vector<float2> tab;
+    kappa_table(tab, c->ktab_u0, c->ktab_idu);
+    c->ktab_n = (int)tab.size();
+    if ((e = hipMalloc((void **)&c->d_ktab, tab.size() * sizeof(float2))) != hipSuccess ||
+        (e = hipMemcpy(c->d_ktab, tab.data(), tab.size() * sizeof(float2),
+                       hipMemcpyHostToDevice)) != hipSuccess) {
+        fail(nullptr, RT_E_HIP, "HIP set-up failed: %s", hipGetErrorString(e));
+        rt_destroy(c);
+        return nullptr;
+    }
     return c;
 }
+
 
 void rt_destroy(rt_ctx *c)
 {
@@ -993,6 +1020,7 @@ void rt_destroy(rt_ctx *c)
         (void)hipStreamSynchronize(c->stream);
     (void)hipFree(c->d_prims);
     (void)hipFree(c->d_queue);
+    (void)hipFree(c->d_ktab);
     for (auto &t : c->orders)
         (void)hipFree(t.d);
     for (auto &b : c->states)
@@ -1429,7 +1457,7 @@ int rt_probe(rt_ctx *c, int kind, const double *h_in, int n, double *h_out)
             hipLaunchKernelGGL(probe_camera, grid, block, 0, c->stream, P, d_in, n, d_out);
             break;
         case RT_PROBE_EMISSION_MODEL:
-            hipLaunchKernelGGL(probe_emission_model, grid, block, 0, c->stream, kappa_const(),
+            hipLaunchKernelGGL(probe_emission_model, grid, block, 0, c->stream, kappa_with_table(c),
                                d_in, n, d_out);
             break;
         default:

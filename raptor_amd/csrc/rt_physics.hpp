@@ -26,6 +26,7 @@
 #include <string.h>
 
 #include <cmath>
+#include <vector>
 
 namespace rt {
 
@@ -79,6 +80,12 @@ struct KappaConst {
     float xa;              // A_s exponent
     float ha, hb, hc;      // hypergeometric parameters kappa - 1/3, kappa + 1, kappa + 2/3
     float g1, g2;          // Gamma-function ratios of the 1/(1-z) continuation
+    // log2 of the absorptivity's hypergeometric factor tabulated over
+    // u = log2(-z) (value and d/du at u0 + i / idu, i < n), built on the host
+    // in FP64 (kappa_table); NULL evaluates the series instead
+    const float2 *tab;
+    float tu0, tidu;
+    int tn;
 };
 
 struct Params {
@@ -851,6 +858,23 @@ __host__ __device__ inline float kappa_hyp(const KappaConst &K, float z)
            fexp2(-K.hb * l1z) * K.g2 * hyp_series(K.hb, K.hc - K.ha, K.hb - K.ha + 1.f, x);
 }
 
+// log2 of kappa_hyp(K, -2^u): cubic Hermite interpolation of the table
+// (value and derivative at both ends of the interval), the series outside it.
+__host__ __device__ inline float kappa_lhyp(const KappaConst &K, float u)
+{
+    const float t = (u - K.tu0) * K.tidu;
+    const float fi = floorf(t);
+    if (K.tab && fi >= 0.f && fi < (float)(K.tn - 1)) {
+        const int i = (int)fi;
+        const float f = t - fi, h = 1.f / K.tidu;
+        const float2 p0 = K.tab[i], p1 = K.tab[i + 1];
+        const float f2 = f * f, g = 1.f - f;
+        return p0.x * ((1.f + 2.f * f) * g * g) + (p0.y * h) * (f * g * g) +
+               p1.x * (f2 * (3.f - 2.f * f)) - (p1.y * h) * (f2 * g);
+    }
+    return flog2(kappa_hyp(K, -fexp2(u)));
+}
+
 // emission_coeff_kappa_FIT and absorption_coeff_kappa_FIT (j_nu.c:28-106,
 // Pandya et al. 2016), kappa = 3.5, width w = Theta_e, mu = cos(pitch).
 // j = (e^2/c) n_e nu_c sin J_s, alpha = n_e e / (B sin) A_s, with the
@@ -874,7 +898,7 @@ __host__ __device__ inline EmisX emission_kappa(const KappaConst &K, float B, fl
     const float z = -wk;
     if (fabsf(z) == 1.f)
         return e;   // j_nu.c:79: the absorptivity is 0 there
-    const float lhyp = flog2(kappa_hyp(K, z));
+    const float lhyp = K.tab ? kappa_lhyp(K, lwk) : flog2(kappa_hyp(K, z));
     const float lAl = -(5.f / 3.f) * lX + K.lAlow - (16.f / 3.f - kap) * lwk + lhyp;
     const float lAh = -0.5f * (3.f + kap) * lX + K.lAhigh - 5.f * lwk;
     const float lA = -log2_add(-K.xa * lAl, -K.xa * lAh) * frcp(K.xa);
@@ -1289,7 +1313,51 @@ inline KappaConst kappa_const()
     K.hc = (float)c;
     K.g1 = (float)(std::tgamma(c) * std::tgamma(b - a) / (std::tgamma(b) * std::tgamma(c - a)));
     K.g2 = (float)(std::tgamma(c) * std::tgamma(a - b) / (std::tgamma(a) * std::tgamma(c - b)));
+    K.tab = nullptr;
+    K.tu0 = 0.f;
+    K.tidu = 1.f;
+    K.tn = 0;
     return K;
+}
+
+// The absorptivity's hypergeometric factor in FP64, by the same two forms as
+// kappa_hyp (Pfaff's transformation for -1 <= z < 0, the 1/(1-z)
+// continuation below), each series summed to 1e-17.
+inline double kappa_hyp_f64(double z)
+{
+    const double k = 3.5, a = k - 1. / 3., b = k + 1., c = k + 2. / 3.;
+    auto series = [](double pa, double pb, double pc, double x) {
+        double term = 1., sum = 1.;
+        for (int n = 0; n < 2000; n++) {
+            term *= (pa + n) * (pb + n) / ((pc + n) * (n + 1.)) * x;
+            sum += term;
+            if (std::fabs(term) <= 1e-17 * std::fabs(sum))
+                break;
+        }
+        return sum;
+    };
+    if (z >= -1.)
+        return std::pow(1. - z, -a) * series(a, c - b, c, z / (z - 1.));
+    const double g1 = std::tgamma(c) * std::tgamma(b - a) / (std::tgamma(b) * std::tgamma(c - a));
+    const double g2 = std::tgamma(c) * std::tgamma(a - b) / (std::tgamma(a) * std::tgamma(c - b));
+    return std::pow(-z, -a) * g1 +
+           std::pow(1. - z, -b) * g2 * series(b, c - a, b - a + 1., 1. / (1. - z));
+}
+
+// The table kappa_lhyp reads: log2 kappa_hyp(-2^u) and its derivative in u
+// (central difference, step 1e-4, in FP64) at u = -32 + i/16, i < 1025.
+inline void kappa_table(std::vector<float2> &tab, float &u0, float &idu)
+{
+    const int n = 1025;
+    u0 = -32.f;
+    idu = 16.f;
+    tab.resize(n);
+    auto g = [](double u) { return std::log2(kappa_hyp_f64(-std::exp2(u))); };
+    for (int i = 0; i < n; i++) {
+        const double u = -32. + i / 16.;
+        tab[i].x = (float)g(u);
+        tab[i].y = (float)((g(u + 1e-4) - g(u - 1e-4)) / 2e-4);
+    }
 }
 
 template <class RtParams, class RtGrid>

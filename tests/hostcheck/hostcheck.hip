@@ -98,6 +98,29 @@ extern "C" int hc_render_split(const rt_params *p, const rt_grid *g, const float
                                const double *incl, int ncam, const double *freqs, int nf,
                                int split, double near_off, double *img, double *dimg, int *cnt);
 
+// The kappa fit's constants with the hypergeometric table, as the library
+// hands them to its kernels.
+static KappaConst host_kappa()
+{
+    static std::vector<float2> tab;
+    static float u0, idu;
+    if (tab.empty())
+        kappa_table(tab, u0, idu);
+    KappaConst K = kappa_const();
+    K.tab = tab.data();
+    K.tu0 = u0;
+    K.tidu = idu;
+    K.tn = (int)tab.size();
+    return K;
+}
+
+// log2 of the kappa absorptivity's hypergeometric factor at z = -2^u, as the
+// kernels evaluate it (table inside its range, series outside)
+extern "C" double hc_kappa_lhyp(double u)
+{
+    return (double)kappa_lhyp(host_kappa(), (float)u);
+}
+
 extern "C" int hc_render(const rt_params *p, const rt_grid *g, const float *soa,
                          const double *incl, int ncam, const double *freqs, int nf, double *img,
                          double *dimg, int *cnt)
@@ -115,6 +138,7 @@ extern "C" int hc_render_split(const rt_params *p, const rt_grid *g, const float
         delete P;
         return -1;
     }
+    P->kc = host_kappa();
     size_t ncell = (size_t)g->n1 * g->n2 * g->n3;
     std::vector<float4> aos(2 * ncell);
     for (size_t c = 0; c < ncell; c++) {
@@ -182,7 +206,7 @@ extern "C" void hc_emission(int model, double B, double mu, double Te, double nu
         out[1] = e.jh != 0.f ? jx * (double)e.kap : (e.poison ? NAN : 0.);
         return;
     }
-    const KappaConst K = kappa_const();
+    const KappaConst K = host_kappa();
     EmisX e = model == kKappa
                   ? emission_kappa(K, (float)B, (float)mu, (float)Te, (float)nu, (float)ne)
                   : emission_thsynchav((float)B, (float)Te, (float)nu, (float)ne);

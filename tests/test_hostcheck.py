@@ -31,6 +31,35 @@ def _lib():
     return ctypes.CDLL(LIB)
 
 
+def test_kappa_hypergeometric_table():
+    """The kappa absorptivity's hypergeometric factor as the kernels evaluate
+    it: log2 of 2F1(kappa - 1/3, kappa + 1; kappa + 2/3; z), z = -2^u, by
+    cubic Hermite interpolation of a table built in FP64 over u in [-32, 32]
+    (the series beyond), against scipy's hyp2f1 at random points, at the
+    table's nodes and midpoints, across z = -1 where its two series forms
+    meet, and outside the table (a wrong node spacing, derivative scale or
+    branch is off by far more than the FP32 tolerance)."""
+    from scipy import special
+    L = _lib()
+    L.hc_kappa_lhyp.restype = ctypes.c_double
+    L.hc_kappa_lhyp.argtypes = [ctypes.c_double]
+    k = 3.5
+    rng = np.random.default_rng(7)
+    nodes = np.arange(-32.0, 32.0, 1.0 / 16.0)
+    us = np.concatenate([rng.uniform(-32, 32, 3000), nodes, nodes + 1.0 / 32.0,
+                         np.linspace(-0.05, 0.05, 41), [-40.0, -33.0, 32.5, 40.0]])
+    # the kernels' argument is the FP32 log2(kappa Theta_e)
+    us = us.astype(np.float32).astype(np.float64)
+    worst = 0.0
+    for u in us:
+        want = np.log2(special.hyp2f1(k - 1 / 3, k + 1, k + 2 / 3, -2.0 ** u))
+        got = L.hc_kappa_lhyp(u)
+        err = abs(got - want) / (1.0 + abs(want))
+        worst = max(worst, err)
+        assert err <= 2e-6, (u, got, want)
+    print("kappa table: worst error %.2e" % worst)
+
+
 def _render(dump, W, incl, freqs, grad=False, R_high=R_HIGH, stepsize=0.03, integrator=0,
             near_off=None, emission=0):
     L = _lib()
