@@ -194,9 +194,12 @@ __device__ inline void load_state(const RayState &S, int i, Ray<NF, GRAD> &R)
 // the grid (approach_ray) and leaves it saved for the trace kernel.  Every
 // ray's approach is about ln(r_cam / r_out) / STEPSIZE steps with no fluid
 // work, so these waves run without the idle lanes they would have beside rays
-// inside the grid.
+// inside the grid.  3 waves per SIMD; RK4's four accelerations per step
+// would spill 40 VGPRs there (six reloads a step), and at 2 waves (205
+// VGPRs) it runs 2% faster.
 template <int INT>
-__global__ __launch_bounds__(64, 3) void approach_kernel(const Params P, const TraceArgs A)
+__global__ __launch_bounds__(64, INT == kRK4 ? 2 : 3) void approach_kernel(const Params P,
+                                                                          const TraceArgs A)
 {
     const int i = blockIdx.x * 64 + threadIdx.x;
     if (i >= A.nsaved)
