@@ -8,12 +8,13 @@
 // The functions restate the reference's FP64 path (parameters.h:13):
 //   camera            metric.c:650-727, GRmath.c:135-184, newtonraphson.c:76-97
 //   step size         integrator.c:109-122
-//   RK2 midpoint      integrator.c:44-70
+//   RK2 midpoint      integrator.c:44-70 (RK4 and velocity Verlet: 13-105)
 //   geodesic accel.   integrator.c:126-146 with metric.c:538-638 (MKS Gamma)
 //   metric            metric.c:94-138, 207-231
 //   fluid sample      raptor_harm_model.c:190-320, grmhd.c:35-71
 //   transfer          radiative_transfer.c:17-119
-//   emissivity        j_nu.c:15, 130-153, 188-200
+//   emissivity        j_nu.c:15, 130-153, 188-200 (thermal synchrotron);
+//                     28-106 (kappa = 3.5), 157-167 (angle-averaged thermal)
 // The connection's contraction with k is formed from the Kerr-Schild
 // structure of the metric instead of filling a 4x4x4 array, and the
 // trigonometry of a point (exp X1, sin/cos theta, sin/cos 2 pi X2) is
