@@ -189,6 +189,30 @@ def test_3d_dump_matches_oracle(tracer):
     assert r.max() < PIX_TOL
 
 
+@pytest.mark.parametrize("nf", [3, 8])
+def test_3d_dump_several_frequencies_match_oracle(tracer, dump3d_small, nf):
+    """The trilinear sampler's multi-frequency kernels (4 and 8 frequencies)
+    on the 3D dump, two inclinations (one nearly edge-on, whose rays cross
+    many phi cells), every frequency against the oracle."""
+    W = 24
+    freqs = list(86e9 * (690.0 / 86.0) ** (np.arange(nf) / max(1, nf - 1)))
+    incl = [INCL, 85.0]
+    _setup(tracer, dump3d_small, W)
+    img, spec = tracer.render(incl, freqs)
+    for c, inc in enumerate(incl):
+        P = O.params(dump3d_small.grid, width=W, height=W, inclination=inc, M_unit=M_UNIT,
+                     R_high=R_HIGH)
+        ora, gate = O.render(P, dump3d_small.prims, freqs, gate=True)
+        for f in range(nf):
+            if ora[:, f].max() <= 0:
+                assert np.all(img[c, :, f] == 0)
+                continue
+            r = _report("3d i=%g nu=%.3g" % (inc, freqs[f]), img[c, :, f], ora[:, f])
+            assert abs(img[c, :, f].sum() / ora[:, f].sum() - 1) < FLUX_TOL
+            assert r[~(gate[:, f] < GATE_EPS)].max() < PIX_TOL
+            assert spec[c, f] == pytest.approx(img[c, :, f].sum(), rel=1e-13)
+
+
 def test_c1_full_image(tracer, dump256):
     """The 1024^2 image of the 256^2 dump against the oracle on every pixel.
     Pixels whose FP64 tau passes within GATE_EPS of the ln 1000 gate
