@@ -137,6 +137,31 @@ def test_gradients_match_oracle(tracer, dump128):
         assert np.mean(r < 1e-3) > 0.99 and r.max() < 1e-2
 
 
+@pytest.mark.parametrize("nf", [3, 8])
+def test_gradients_several_frequencies_match_oracle(tracer, dump128, nf):
+    """The tangent pass of the 4- and 8-frequency kernels against the
+    oracle's tangents, every frequency.  Pixels near the ln 1000 gate are held
+    apart per pixel, as for the images.  At 86 GHz the torus is thick enough
+    that some pixels hardly depend on M_unit: their tangent, a sum of terms
+    of both signs, is ~1e-5 of the image's largest, and its relative error
+    says little; every pixel is held to 1e-3 in the logarithmic derivative
+    d ln I / d ln p instead, 99% to 1e-3 relative, and the sums to 1e-4."""
+    P = _setup(tracer, dump128, 32)
+    freqs = list(86e9 * (690.0 / 86.0) ** (np.arange(nf) / (nf - 1)))
+    img, dimg, _ = tracer.render([INCL], freqs, grad=True)
+    oimg, odimg, gate = O.render(P, dump128.prims, freqs, grad=True, gate=True)
+    for f in range(nf):
+        off = ~(gate[:, f] < GATE_EPS)
+        assert rel_err(img[0, :, f], oimg[:, f])[off].max() < PIX_TOL, f
+        ipix = np.maximum(oimg[:, f], 1e-6 * oimg[:, f].max())
+        for q, p in enumerate((1.0, R_HIGH)):   # d/d ln M_unit, R_high d/d R_high
+            g, o = dimg[0, :, f, q], odimg[:, f, q]
+            assert abs(g.sum() / o.sum() - 1) < 1e-4, (f, q)
+            r = rel_err(g, o, 1e-5)[off]
+            assert np.mean(r < 1e-3) > 0.99, (f, q)
+            assert (np.abs(g - o) * p / ipix)[off].max() < 1e-3, (f, q)
+
+
 def test_multi_frequency_and_camera_batch(tracer, dump128):
     """One launch over 3 inclinations x 3 frequencies equals the per-image launches."""
     _setup(tracer, dump128, 24)
