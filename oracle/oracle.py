@@ -80,6 +80,8 @@ def lib():
         L.or_units.argtypes = [P, _dp]
         L.or_fluid_at.argtypes = [P, _dp, _dp, _dp]
         L.or_fluid_at.restype = ctypes.c_int
+        L.or_sample_at.argtypes = [P, _dp, _dp, ctypes.c_int, _dp, _dp]
+        L.or_sample_at.restype = ctypes.c_int
         L.or_ray_path.argtypes = [P, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int]
         L.or_ray_path.restype = ctypes.c_int
         L.or_render.argtypes = [P, _dp, ctypes.c_int, _dp, _ip, ctypes.c_int, _dp, _dp, _ip,
@@ -167,6 +169,19 @@ def fluid_at(P, prims, X):
     ok = lib().or_fluid_at(ctypes.byref(P), _d(p), _d(np.ascontiguousarray(X, float)), _d(out))
     return dict(ne=out[0], thetae=out[1], B=out[2], ucon=out[3:7], bcon=out[7:11],
                 dthetae_dRhigh=out[11], ok=bool(ok))
+
+
+def sample_at(P, prims, y, freqs):
+    """One transfer sample at y = (X, k) composed as the oracle's ray loop
+    composes it: dict(ok, ne, thetae, B, mu, nu_p[nf], j[nf], alpha[nf])."""
+    freqs = np.ascontiguousarray(np.atleast_1d(freqs), float)
+    out = np.zeros(5 + 3 * freqs.size)
+    p = np.ascontiguousarray(prims, float)
+    lib().or_sample_at(ctypes.byref(P), _d(p), _d(np.ascontiguousarray(y, float)), freqs.size,
+                       _d(freqs), _d(out))
+    q = out[5:].reshape(-1, 3)
+    return dict(ok=bool(out[0]), ne=out[1], thetae=out[2], B=out[3], mu=out[4],
+                nu_p=q[:, 0], j=q[:, 1], alpha=q[:, 2])
 
 
 def ray_path(P, alpha, beta, maxpts=200000):

@@ -714,6 +714,56 @@ void or_transfer_step(double *tau, double *I, double j, double an, double nu_p, 
     *dI_out = dI;
 }
 
+/* ------------------------------------------- plasma-frame quantities */
+/* GRmath.c:17-27 lower_index: k_a = g_ab k^b */
+static void or_lower(const double gcov[16], const double k[4], double kd[4])
+{
+    for (int a = 0; a < 4; a++) {
+        kd[a] = 0.;
+        for (int b = 0; b < 4; b++)
+            kd[a] += gcov[a * 4 + b] * k[b];
+    }
+}
+
+/* radiative_transfer.c:32-47 pitch_angle (the cosine), with k lowered as at
+ * radiative_transfer.c:66: (k_a b^a) / (|k_a u^a| |b|), |b| = B / B_unit,
+ * clamped to +-1 by mu / |mu|; 0 for B == 0. */
+static double or_pitch_cos(const double gcov[16], const double k[4], const or_fluid_t *F,
+                           double B_unit)
+{
+    if (F->B == 0.)
+        return 0.;
+    double kd[4];
+    or_lower(gcov, k, kd);
+    double kk = fabs(kd[0] * F->ucon[0] + kd[1] * F->ucon[1] + kd[2] * F->ucon[2] +
+                     kd[3] * F->ucon[3]);
+    double mu = (kd[0] * F->bcon[0] + kd[1] * F->bcon[1] + kd[2] * F->bcon[2] +
+                 kd[3] * F->bcon[3]) / (kk * F->B / B_unit);
+    if (fabs(mu) > 1.)
+        mu /= fabs(mu);
+    return mu;
+}
+
+/* radiative_transfer.c:17-29 freq_in_plasma_frame with the wave vector scaled
+ * to frequency nu and lowered (radiative_transfer.c:74-79):
+ * nu_p = -(u^a k_a) m_e c^2 / h with k = k_geo h nu / (m_e c^2); nu_p <= 0
+ * becomes 1 Hz. */
+static double or_nu_plasma(const double gcov[16], const double k[4], const double ucon[4],
+                           double nu)
+{
+    double kuf[4], kdf[4];
+    for (int a = 0; a < 4; a++)
+        kuf[a] = k[a] * OR_PLANCK * nu / (OR_ELECTRON_MASS * OR_C * OR_C);
+    or_lower(gcov, kuf, kdf);
+    double nu_p = 0.;
+    for (int a = 0; a < 4; a++)
+        nu_p += ucon[a] * kdf[a];
+    nu_p *= -(OR_ELECTRON_MASS * OR_C * OR_C) / OR_PLANCK;
+    if (nu_p <= 0.0)
+        nu_p = 1.0;
+    return nu_p;
+}
+
 /* ------------------------------------------------------------ one ray */
 /* integrator.c:149-289 integrate_geodesic + radiative_transfer.c:49-119.
  * I[f] returns the invariant intensity I_nu/nu^3 before the nu^3 scaling of
@@ -744,7 +794,6 @@ void or_trace_ray(const or_params *P, const double *prims, int x, int yy, int nf
     double gate[64];
     for (int f = 0; f < nf; f++)
         gate[f] = INFINITY;
-    const double mc2_h = OR_ELECTRON_MASS * OR_C * OR_C / OR_PLANCK;
 
     while (r_current < OR_CUTOFF_OUTER && r_current > D.cutoff_inner &&
            steps < P->max_steps && !TERMINATE) {
@@ -771,36 +820,9 @@ void or_trace_ray(const or_params *P, const double *prims, int x, int yy, int nf
             if (or_fluid(P, &D, prims, y, gcov, gcon, &F)) {
                 n_rt++;
                 const double *k = y + 4;
-                double kd[4] = {0, 0, 0, 0};
-                for (int a = 0; a < 4; a++)
-                    for (int b = 0; b < 4; b++)
-                        kd[a] += gcov[a * 4 + b] * k[b];
-                /* radiative_transfer.c:32-47 pitch_angle (returns cos) */
-                double mu;
-                if (F.B == 0.) {
-                    mu = 0;
-                } else {
-                    double kk = fabs(kd[0] * F.ucon[0] + kd[1] * F.ucon[1] + kd[2] * F.ucon[2] +
-                                     kd[3] * F.ucon[3]);
-                    mu = (kd[0] * F.bcon[0] + kd[1] * F.bcon[1] + kd[2] * F.bcon[2] +
-                          kd[3] * F.bcon[3]) / (kk * F.B / D.B_unit);
-                    if (fabs(mu) > 1.)
-                        mu /= fabs(mu);
-                }
+                const double mu = or_pitch_cos(gcov, k, &F, D.B_unit);
                 for (int f = 0; f < nf; f++) {
-                    double kuf[4], kdf[4] = {0, 0, 0, 0};
-                    for (int a = 0; a < 4; a++)
-                        kuf[a] = k[a] * OR_PLANCK * freqs[f] / (OR_ELECTRON_MASS * OR_C * OR_C);
-                    for (int a = 0; a < 4; a++)
-                        for (int b = 0; b < 4; b++)
-                            kdf[a] += gcov[a * 4 + b] * kuf[b];
-                    /* radiative_transfer.c:17-29 */
-                    double nu_p = 0.;
-                    for (int a = 0; a < 4; a++)
-                        nu_p += F.ucon[a] * kdf[a];
-                    nu_p *= -mc2_h;
-                    if (nu_p <= 0.0)
-                        nu_p = 1.0;
+                    const double nu_p = or_nu_plasma(gcov, k, F.ucon, freqs[f]);
                     double j, an;
                     or_emission(P->emission, F.B, mu, F.thetae, nu_p, F.ne, &j, &an);
                     double dlf = dlabs * D.Rg / freqs[f];
@@ -928,6 +950,39 @@ int or_fluid_at(const or_params *P, const double *prims, const double X[4], doub
     }
     out[11] = F.dthetae_dRhigh;
     out[12] = ok;
+    return ok;
+}
+
+/* one transfer sample for tests, composed as or_trace_ray composes it
+ * (radiative_transfer.c:61-84): at y = (X, k) the fluid state, the pitch
+ * angle's cosine and, per frequency, the plasma-frame frequency and the
+ * emission coefficients.  out = {ok, ne, thetae, B, mu, then per frequency
+ * nu_p, j_nu, alpha_nu}; ok = 0 where the sigma cut skips the sample (the
+ * rest is then as computed, for inspection). */
+int or_sample_at(const or_params *P, const double *prims, const double y[8], int nf,
+                 const double *freqs, double *out)
+{
+    or_derived D;
+    or_derive(P, &D);
+    double gcov[16], gcon[16];
+    or_metric_dd(P, y, gcov);
+    or_metric_uu(P, y, gcon);
+    or_fluid_t F;
+    int ok = or_fluid(P, &D, prims, y, gcov, gcon, &F);
+    const double mu = or_pitch_cos(gcov, y + 4, &F, D.B_unit);
+    out[0] = ok;
+    out[1] = F.ne;
+    out[2] = F.thetae;
+    out[3] = F.B;
+    out[4] = mu;
+    for (int f = 0; f < nf; f++) {
+        const double nu_p = or_nu_plasma(gcov, y + 4, F.ucon, freqs[f]);
+        double j, an;
+        or_emission(P->emission, F.B, mu, F.thetae, nu_p, F.ne, &j, &an);
+        out[5 + 3 * f] = nu_p;
+        out[6 + 3 * f] = j;
+        out[7 + 3 * f] = an;
+    }
     return ok;
 }
 

@@ -321,6 +321,168 @@ def test_sigma_cut():
     assert not O.fluid_at(P, _uniform_prims(g, 1.0, 0.05, 10.0), X)["ok"]
 
 
+# ------------------------------------------------- plasma-frame quantities
+#
+# The transfer sample's Doppler factor nu_p / nu and pitch-angle cosine
+# (radiative_transfer.c:17-47, composed at :61-84) against an independent
+# construction: the Kerr-Schild metric written here in closed form and pulled
+# back to MKS, the fluid 4-velocity built as u = gamma n + u~ from the normal
+# observer n (HARM's primitive velocity), and the angle measured in an
+# orthonormal fluid-frame tetrad.  A mistake in the oracle's lowering of k
+# (the inverse metric, a transposed index), a sign of nu_p, a dropped
+# m_e c^2 / h or B_unit, or u^mu in place of u_mu fails these.
+
+
+def _mks_metric_np(X, a, h):
+    r = math.exp(X[1])
+    th = math.pi * X[2] + 0.5 * (1 - h) * math.sin(2 * math.pi * X[2])
+    dth = math.pi * (1 + (1 - h) * math.cos(2 * math.pi * X[2]))
+    s2, c2 = math.sin(th) ** 2, math.cos(th) ** 2
+    S = r * r + a * a * c2
+    z = 2 * r / S
+    g = np.zeros((4, 4))
+    g[0, 0] = -(1 - z)
+    g[0, 1] = g[1, 0] = z
+    g[0, 3] = g[3, 0] = -z * a * s2
+    g[1, 1] = 1 + z
+    g[1, 3] = g[3, 1] = -a * s2 * (1 + z)
+    g[2, 2] = S
+    g[3, 3] = s2 * (S + a * a * s2 * (1 + z))
+    J = np.diag([1.0, r, dth, 1.0])
+    return J @ g @ J
+
+
+def _null_k(g, kspace):
+    """k^t > 0 with g(k, k) = 0 for the spatial components kspace."""
+    ks = np.asarray(kspace, float)
+    A = g[0, 0]
+    B = 2 * g[0, 1:] @ ks
+    Cq = ks @ g[1:, 1:] @ ks
+    kt = (-B - math.sqrt(B * B - 4 * A * Cq)) / (2 * A)
+    if kt < 0:
+        kt = (-B + math.sqrt(B * B - 4 * A * Cq)) / (2 * A)
+    return np.concatenate([[kt], ks])
+
+
+def _fluid_np(g, vt, Bp):
+    """u = gamma n + u~ (n the normal observer, gamma^2 = 1 + g_ij u~^i u~^j)
+    and b^t = u_i B^i, b^i = (B^i + u^i b^t) / u^t."""
+    gi = np.linalg.inv(g)
+    alpha = 1.0 / math.sqrt(-gi[0, 0])
+    n = -alpha * gi[:, 0]
+    vt = np.asarray(vt, float)
+    gamma = math.sqrt(1.0 + vt @ g[1:, 1:] @ vt)
+    u = gamma * n + np.concatenate([[0.0], vt])
+    ul = g @ u
+    bt = ul[1:] @ np.asarray(Bp, float)
+    b = np.concatenate([[bt], (np.asarray(Bp, float) + u[1:] * bt) / u[0]])
+    return u, b
+
+
+def _tetrad_cos(g, u, k, b):
+    """cos of the angle between the photon's direction and the field, both
+    expressed in an orthonormal tetrad whose time leg is u (Gram-Schmidt of
+    the coordinate legs d/dX1, d/dX2, d/dX3 orthogonal to u)."""
+    dot = lambda p, q: p @ g @ q
+    legs = []
+    for i in (1, 2, 3):
+        v = np.zeros(4)
+        v[i] = 1.0
+        v = v + dot(v, u) * u
+        for e in legs:
+            v = v - dot(v, e) * e
+        legs.append(v / math.sqrt(dot(v, v)))
+    kk = np.array([dot(k, e) for e in legs])
+    bb = np.array([dot(b, e) for e in legs])
+    return kk @ bb / (np.linalg.norm(kk) * np.linalg.norm(bb)), np.linalg.norm(bb)
+
+
+def _uniform_fluid(grid, vt, Bp, rho=1.0, uu=0.05):
+    p = np.zeros((8, grid.n1, grid.n2, 1))
+    p[0], p[1] = rho, uu
+    p[2], p[3], p[4] = vt
+    p[5], p[6], p[7] = Bp
+    return p
+
+
+def _random_points(grid, rng, n):
+    for _ in range(n):
+        X = np.array([0.0, rng.uniform(grid.startx1 + 2 * grid.dx1, grid.stopx1 - 2 * grid.dx1),
+                      rng.uniform(0.05, 0.95), rng.uniform(0, 2 * math.pi)])
+        ks = rng.normal(size=3) * np.array([0.3, 0.05, 0.2])
+        yield X, ks
+
+
+def test_doppler_factor_of_the_normal_observer():
+    """With u~ = 0 the fluid is the normal observer, u_mu = (-alpha, 0, 0, 0),
+    so nu_p / nu = -u.k = alpha k^t = k^t / sqrt(-g^tt), g^tt = -(1 + 2r/Sigma)
+    in (M)KS; at several frequencies and spins."""
+    rng = np.random.default_rng(11)
+    for a in (0.9375, 0.0, -0.5):
+        g0 = _grid(a=a, n1=16, n2=16)
+        P = O.params(g0, width=4, height=4, inclination=60.0, M_unit=1e23, R_high=10.0)
+        prims = _uniform_fluid(g0, (0, 0, 0), (1e-3, 0, 0))
+        for X, ks in _random_points(g0, rng, 40):
+            g = _mks_metric_np(X, a, g0.hslope)
+            k = _null_k(g, ks)
+            r = math.exp(X[1])
+            th = math.pi * X[2] + 0.5 * (1 - g0.hslope) * math.sin(2 * math.pi * X[2])
+            S = r * r + a * a * math.cos(th) ** 2
+            want = k[0] / math.sqrt(1 + 2 * r / S)
+            s = O.sample_at(P, prims, np.concatenate([X, k]), [2.3e11, 8.6e10])
+            assert s["ok"]
+            assert s["nu_p"] / np.array([2.3e11, 8.6e10]) == pytest.approx(
+                [want, want], rel=1e-12)
+
+
+def test_doppler_factor_of_a_moving_fluid():
+    """For HARM's primitive velocity u~ the fluid moves with u = gamma n + u~;
+    nu_p / nu = -u_mu k^mu with u lowered by the closed-form metric, and the
+    1 Hz floor where that is <= 0 (radiative_transfer.c:24-26)."""
+    rng = np.random.default_rng(12)
+    g0 = _grid(n1=16, n2=16)
+    P = O.params(g0, width=4, height=4, inclination=60.0, M_unit=1e23, R_high=10.0)
+    n = 0
+    for vt in ((0.3, -0.02, 0.1), (-0.5, 0.05, 0.4), (0.05, 0.0, -0.3)):
+        prims = _uniform_fluid(g0, vt, (1e-3, 2e-4, -1e-4))
+        for X, ks in _random_points(g0, rng, 30):
+            g = _mks_metric_np(X, g0.a, g0.hslope)
+            k = _null_k(g, ks)
+            u, _ = _fluid_np(g, vt, (0, 0, 0))
+            want = -(g @ u) @ k
+            got = O.sample_at(P, prims, np.concatenate([X, k]), [2.3e11])["nu_p"][0] / 2.3e11
+            if want > 0:
+                assert got == pytest.approx(want, rel=1e-11)
+                n += 1
+            else:
+                assert got == 1.0 / 2.3e11
+    assert n > 60
+
+
+def test_pitch_angle_in_the_fluid_frame():
+    """mu = cos of the angle between the photon and b measured in the fluid's
+    rest frame (an orthonormal tetrad with time leg u), for random fluid
+    velocities, fields and photon directions; and |b| there times B_unit is
+    the oracle's B (the field strength the emissivity uses)."""
+    rng = np.random.default_rng(13)
+    g0 = _grid(n1=16, n2=16)
+    P = O.params(g0, width=4, height=4, inclination=60.0, M_unit=1e23, R_high=10.0)
+    B_unit = O.units(P)["B_unit"]
+    for _ in range(6):
+        vt = rng.normal(size=3) * np.array([0.3, 0.03, 0.3])
+        Bp = rng.normal(size=3) * np.array([1e-3, 1e-4, 1e-3])
+        prims = _uniform_fluid(g0, vt, Bp)
+        for X, ks in _random_points(g0, rng, 20):
+            g = _mks_metric_np(X, g0.a, g0.hslope)
+            k = _null_k(g, ks)
+            u, b = _fluid_np(g, vt, Bp)
+            want, bnorm = _tetrad_cos(g, u, k, b)
+            s = O.sample_at(P, prims, np.concatenate([X, k]), [2.3e11])
+            assert s["ok"]
+            assert s["mu"] == pytest.approx(want, abs=1e-10)
+            assert s["B"] == pytest.approx(bnorm * B_unit, rel=1e-10)
+
+
 def test_units_closed_form():
     P = _P(M_unit=3e21)
     u = O.units(P)
@@ -440,6 +602,59 @@ def test_kappa_fits_against_distribution_integrals():
     for w in {r["w"] for r in same}:
         a, b = [_kappa_fit(r) for r in same if r["w"] == w]
         assert a == pytest.approx(b, rel=1e-12)
+
+
+def test_kappa_fits_asymptotic_slopes_and_scalings():
+    """What the kappa distribution fixes exactly in the fits of j_nu.c:28-106
+    (Pandya et al. 2016, kappa = 3.5), independent of their fitted constants:
+    the power-law slopes in X = nu / nu_s, nu_s = nu_c sin(theta) (w kappa)^2
+    -- emissivity X^(1/3) below the peak and X^(-(kappa-2)/2) above it,
+    absorptivity X^(-5/3) and X^(-(kappa+3)/2) -- and the scalings at fixed X:
+    j proportional to n_e nu_c sin(theta) with no other dependence on the
+    width w; alpha proportional to n_e / (B sin(theta)) times (w kappa)^-5 in
+    the high-X regime and (w kappa)^(kappa - 16/3) 2F1(kappa - 1/3, kappa + 1;
+    kappa + 2/3; -kappa w) in the low-X regime (2F1 from scipy).  A wrong
+    exponent, a swapped regime, X formed with the wrong power of w or a
+    dropped hypergeometric factor fails one of these.  The fits' constant
+    factors are pinned only by the distribution integrals above."""
+    from scipy.special import hyp2f1
+    lib = O.lib()
+    kap = 3.5
+
+    def nu_of(X, w, B, mu):
+        nuc = E * B / (2 * math.pi * ME * C)
+        return X * nuc * math.sqrt(1 - mu * mu) * (w * kap) ** 2
+
+    B, mu, ne = 30.0, 0.4, 1e6
+
+    def slope(fn, X, w, h=1e-3):
+        lo = fn(nu_of(X * math.exp(-h), w, B, mu), ne, w, B, mu)
+        hi = fn(nu_of(X * math.exp(h), w, B, mu), ne, w, B, mu)
+        return (math.log(hi) - math.log(lo)) / (2 * h)
+
+    for w in (3.0, 30.0):
+        assert slope(lib.or_j_kappa, 1e-12, w) == pytest.approx(1 / 3, abs=2e-3)
+        assert slope(lib.or_j_kappa, 1e12, w) == pytest.approx(-(kap - 2) / 2, abs=2e-3)
+        assert slope(lib.or_a_kappa, 1e-12, w) == pytest.approx(-5 / 3, abs=2e-3)
+        assert slope(lib.or_a_kappa, 1e14, w) == pytest.approx(-(kap + 3) / 2, abs=2e-3)
+    hyp = lambda w: hyp2f1(kap - 1 / 3, kap + 1, kap + 2 / 3, -kap * w)
+    for X in (1e-12, 1.0, 1e12):
+        for (B1, mu1, w1), (B2, mu2, w2) in (((30.0, 0.4, 3.0), (7.0, -0.8, 40.0)),
+                                             ((2.0, 0.1, 0.5), (90.0, 0.6, 12.0))):
+            j1 = lib.or_j_kappa(nu_of(X, w1, B1, mu1), ne, w1, B1, mu1)
+            j2 = lib.or_j_kappa(nu_of(X, w2, B2, mu2), 3 * ne, w2, B2, mu2)
+            s1, s2 = math.sqrt(1 - mu1 ** 2), math.sqrt(1 - mu2 ** 2)
+            assert j2 / j1 == pytest.approx(3 * (B2 * s2) / (B1 * s1), rel=1e-12)
+    for X, w1, w2 in ((1e14, 3.0, 40.0), (1e14, 0.5, 12.0)):
+        a1 = lib.or_a_kappa(nu_of(X, w1, B, mu), ne, w1, B, mu)
+        a2 = lib.or_a_kappa(nu_of(X, w2, 2 * B, -0.8), 5 * ne, w2, 2 * B, -0.8)
+        sin_ratio = math.sqrt(1 - mu * mu) / 0.6
+        assert a2 / a1 == pytest.approx(5 / 2 * sin_ratio * (w2 / w1) ** -5, rel=2e-3)
+    for X, w1, w2 in ((1e-14, 3.0, 40.0), (1e-14, 0.5, 12.0)):
+        a1 = lib.or_a_kappa(nu_of(X, w1, B, mu), ne, w1, B, mu)
+        a2 = lib.or_a_kappa(nu_of(X, w2, B, mu), ne, w2, B, mu)
+        want = (w2 / w1) ** (kap - 16 / 3) * hyp(w2) / hyp(w1)
+        assert a2 / a1 == pytest.approx(want, rel=2e-3)
 
 
 def test_kappa_gates():
