@@ -130,12 +130,14 @@ int rt_render_grad(rt_ctx *ctx, const double *inclinations, int n_cam,
                    const double *freqs, int n_freq, double *h_image, double *h_dimage,
                    double *h_spectrum);
 
-/* Device-resident variant: writes d_image (and d_dimage if non-NULL, and
- * d_counters [n_cam*W*H] if non-NULL) in HBM on the context's stream and
- * returns without synchronising.  The buffers must hold the sizes above. */
+/* Device-resident variant: writes d_image (and d_dimage if non-NULL,
+ * d_counters [n_cam*W*H] if non-NULL, and d_spectrum [n_cam][n_freq] -- the
+ * energy spectrum of core.c:146-150, the sum of each image over its pixels --
+ * if non-NULL) in HBM on the context's stream and returns without
+ * synchronising.  The buffers must hold the sizes above. */
 int rt_render_device(rt_ctx *ctx, const double *inclinations, int n_cam,
                      const double *freqs, int n_freq, double *d_image, double *d_dimage,
-                     rt_ray_counters *d_counters);
+                     rt_ray_counters *d_counters, double *d_spectrum);
 
 /* Wait for the context's stream. */
 int rt_synchronize(rt_ctx *ctx);
@@ -143,6 +145,16 @@ int rt_synchronize(rt_ctx *ctx);
 /* Duration in ms of the last render's trace kernel(s), from HIP events on
  * the context's stream (valid after rt_synchronize). */
 double rt_last_kernel_ms(const rt_ctx *ctx);
+
+/* Durations in ms of the last eager render's kernels, from HIP events on the
+ * context's stream (synchronises it): out[0] the approach kernel with the
+ * queue reset (about 0 when no approach ran), out[1] the persistent trace
+ * kernel, out[2] the FP64 re-trace of the rays at the tau gate, out[3] the
+ * spectrum reduction (about 0 when none was asked for).  Kept for renders of
+ * one launch chunk (at most 2^23 rays); RT_E_STATE otherwise.  No
+ * counterpart in the reference, which times its pixel loop with clock()
+ * (core.c:91-93,116). */
+int rt_last_kernel_times(rt_ctx *ctx, double *out);
 
 /* JANSKY_FACTOR of utilities.c:32-42 for the current parameters. */
 double rt_jansky_factor(const rt_ctx *ctx);

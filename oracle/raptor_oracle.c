@@ -84,6 +84,7 @@ typedef struct {
  * by more than that margin adds or drops one sample's e^-6.9-weighted term. */
 typedef struct {
     int steps, n_grid, n_rt, escaped;
+    int grid_step;   /* steps taken before the first one that lands inside the grid, or -1 */
     double gate[64];
 } or_counters;
 
@@ -789,7 +790,7 @@ void or_trace_ray(const or_params *P, const double *prims, int x, int yy, int nf
     double r_current = exp(y[1]);
     int steps = 0, TERMINATE = 0, theta_turns = 0;
     double thetadot_prev = 0.;
-    int n_grid = 0, n_rt = 0, escaped = 0;
+    int n_grid = 0, n_rt = 0, escaped = 0, grid_step = -1;
     const double lnth = log(1000.);
     double gate[64];
     for (int f = 0; f < nf; f++)
@@ -812,6 +813,8 @@ void or_trace_ray(const or_params *P, const double *prims, int x, int yy, int nf
         }
         r_current = exp(y[1]);
         if (y[1] < D.stopx1 && y[1] > P->startx1 && y[2] > P->startx2 && y[2] < D.stopx2) {
+            if (grid_step < 0)
+                grid_step = steps;
             n_grid++;
             double gcov[16], gcon[16];
             or_metric_dd(P, y, gcov);
@@ -865,6 +868,7 @@ void or_trace_ray(const or_params *P, const double *prims, int x, int yy, int nf
         cnt->n_grid = n_grid;
         cnt->n_rt = n_rt;
         cnt->escaped = escaped;
+        cnt->grid_step = grid_step;
         for (int f = 0; f < nf; f++)
             cnt->gate[f] = gate[f];
     }
@@ -872,7 +876,7 @@ void or_trace_ray(const or_params *P, const double *prims, int x, int yy, int nf
 
 /* core.c:74-152 calculate_image over a pixel list.
  * image[p*nf + f] = I_nu (after the nu^3 scaling of core.c:136) for pixel
- * pix[p] (index y*W + x); dimage (optional) [p][f][2]; counters optional [p][4];
+ * pix[p] (index y*W + x); dimage (optional) [p][f][2]; counters optional [p][5];
  * gate (optional) [p][f]: the tau-gate margin of or_counters.
  * nthreads <= 0 uses the OpenMP default. */
 int or_render(const or_params *P, const double *prims, int nf, const double *freqs,
@@ -905,10 +909,11 @@ int or_render(const or_params *P, const double *prims, int nf, const double *fre
             }
         }
         if (counters) {
-            counters[p * 4 + 0] = c.steps;
-            counters[p * 4 + 1] = c.n_grid;
-            counters[p * 4 + 2] = c.n_rt;
-            counters[p * 4 + 3] = c.escaped;
+            counters[p * 5 + 0] = c.steps;
+            counters[p * 5 + 1] = c.n_grid;
+            counters[p * 5 + 2] = c.n_rt;
+            counters[p * 5 + 3] = c.escaped;
+            counters[p * 5 + 4] = c.grid_step;
         }
         if (gate)
             for (int f = 0; f < nf; f++)

@@ -72,6 +72,10 @@ struct TraceArgs {
     double *dimage;
     rt_ray_counters *counters;
     RayState st;        // the saved rays
+    // rays to trace again with the FP64 transfer (exact_kernel): their work
+    // items' indices within the launch, xcount of them
+    int *xlist;
+    unsigned int *xcount;
 };
 
 template <int NF, bool GRAD, bool DIM3>
@@ -305,6 +309,11 @@ __device__ __forceinline__ void trace_rays(const Params &P, const TraceArgs &A)
             }
             if (!cont) {
                 write_ray<NF, GRAD, DIM3>(P, A, ray, R);
+                // its FP32 optical depth met the ln 1000 gate where a
+                // different decision would show in the pixel: listed for
+                // exact_kernel, which traces it again with the FP64 transfer
+                if (needs_exact<EMIS>(R))
+                    A.xlist[atomicAdd(A.xcount, 1u)] = slot;
                 active = false;
             }
         }
@@ -349,7 +358,86 @@ __global__ __launch_bounds__(64, 3) void trace_kernel_3w(const Params P, const T
     trace_rays<NF, GRAD, DIM3, INT, RESUME, EMIS>(P, A);
 }
 
-__global__ void reset_queue_kernel(unsigned int *q) { *q = 0u; }
+// The rays a trace kernel listed (needs_exact) traced again from their start
+// (the camera, or the saved ray at the grid's edge) with the FP64 transfer of
+// the thermal fit, and written over their FP32 outputs.  A kernel of its own:
+// its FP64 arithmetic needs more registers than the trace kernel can give
+// without losing its third wave (in that kernel it cost 6-60 spilled VGPRs).
+// About 1 ray in 3000 at 1024^2 and 230 GHz; consecutive entries go to
+// different waves, so that those few rays run on different SIMDs.
+template <int NF, bool GRAD, bool DIM3, int INT, bool RESUME>
+__global__ __launch_bounds__(64) void exact_kernel(const Params P, const TraceArgs A)
+{
+    const unsigned n = *A.xcount;
+    const unsigned stride = gridDim.x * 64u;
+    for (unsigned i = blockIdx.x + gridDim.x * threadIdx.x; i < n; i += stride) {
+        const int q = A.xlist[i];
+        int ray;
+        if (RESUME && q < A.nsaved)
+            ray = state_ray(A.st, q);
+        else if (!work_to_ray(P, A, A.w0 + q, ray))
+            continue;
+        Ray<NF, GRAD> R;
+        start_ray<RESUME>(P, A, ray, q, R);
+        while (ray_alive<NF, GRAD>(P, R) &&
+               ray_step<NF, GRAD, DIM3, INT, kThSynch, true>(P, A.prims, R)) {
+        }
+        write_ray<NF, GRAD, DIM3>(P, A, ray, R);
+    }
+}
+
+// The energy spectrum (core.c:146-150): per camera and frequency the sum of
+// the image over its pixels, in two passes of fixed order -- partial sums of
+// kSpecBlocks contiguous pixel ranges, then their sum -- so that every call
+// gives the same bits.  In the first pass thread t of a block sums frequency
+// t % nf over every (256 / nf)-th pixel of the block's range (its reads of
+// image[pixel][f] are contiguous across the block), then one thread per
+// frequency adds those up in order.
+static constexpr int kSpecBlocks = 256;
+
+__global__ __launch_bounds__(256) void spectrum_partial(const double *img, int npix, int nf,
+                                                        double *part)
+{
+    __shared__ double red[256];
+    const int cam = blockIdx.y, nb = gridDim.x;
+    const int per = (npix + nb - 1) / nb;
+    const int p0 = blockIdx.x * per, p1 = min(npix, p0 + per);
+    const int per_f = 256 / nf;   // threads per frequency
+    const int t = threadIdx.x;
+    double sum = 0.;
+    if (t < per_f * nf) {
+        const double *im = img + (size_t)cam * npix * nf;
+        const int f = t % nf;
+        for (int i = p0 + t / nf; i < p1; i += per_f)
+            sum += im[(size_t)i * nf + f];
+    }
+    red[t] = sum;
+    __syncthreads();
+    if (t < nf) {
+        double s = 0.;
+        for (int k = 0; k < per_f; k++)
+            s += red[t + k * nf];
+        part[((size_t)cam * nb + blockIdx.x) * nf + t] = s;
+    }
+}
+
+__global__ void spectrum_final(const double *part, int nb, int nf, double *spec)
+{
+    const int cam = blockIdx.x, f = threadIdx.x;
+    if (f >= nf)
+        return;
+    double s = 0.;
+    for (int b = 0; b < nb; b++)
+        s += part[((size_t)cam * nb + b) * nf + f];
+    spec[cam * nf + f] = s;
+}
+
+// q[0]: the trace kernel's claim counter, q[1]: the length of its exact list
+__global__ void reset_queue_kernel(unsigned int *q)
+{
+    q[0] = 0u;
+    q[1] = 0u;
+}
 
 // ---- probes: one thread per point ----------------------------------------
 __global__ void probe_metric(const Params P, const double *in, int n, double *out)
@@ -498,6 +586,10 @@ struct rt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // boundaries between a one-chunk render's kernels: after the approach,
+    // after the trace kernel, after the FP64 re-trace (rt_last_kernel_times)
+    hipEvent_t evA = nullptr, evT = nullptr, evX = nullptr;
+    int last_chunks = 0;   // launch chunks of the last eager render
     int num_cu = 0;
     bool have_params = false, have_grid = false;
     rt_params params{};
@@ -525,6 +617,13 @@ struct rt_ctx {
     };
     std::vector<StateBuf> states;
     std::vector<void *> capture_pins;   // tables and buffers used by the open capture
+    // work items of one launch to trace again with the FP64 transfer
+    // (exact_kernel), kMaxChunk of them: made by the first eager render and
+    // kept, so that a captured graph's launches always find it
+    int *d_xlist = nullptr;
+    // spectrum: partial sums [camera][kSpecBlocks][frequency] and, for host
+    // renders, the spectrum itself; made by rt_create
+    double *d_spec_part = nullptr, *d_spec = nullptr;
     double *d_image = nullptr, *d_dimage = nullptr;
     size_t image_cap = 0, dimage_cap = 0;
     // the kappa absorptivity's hypergeometric table (kappa_table), made once
@@ -667,35 +766,71 @@ static hipError_t dispatch_nf(rt_ctx *c, const Params &P, const TraceArgs &A)
     return launch_trace<16, GRAD, DIM3, INT, RESUME>(c, P, A);
 }
 
-// All work items in one persistent launch (pct < 0), or chunk by chunk: the
-// approach kernel over the first pct percent of the chunk's work items, then
-// the persistent kernel, which resumes those and traces the rest from the
-// camera.  A chunk saves at most A.st.n rays.
+// The FP64 re-trace after a trace kernel (thermal fit only): a grid of up to
+// 4 workgroups per CU, each lane taking list entries in a grid-sized stride.
+template <int NF, bool GRAD, bool DIM3, int INT, bool RESUME>
+static hipError_t launch_exact(rt_ctx *c, const Params &P, const TraceArgs &A)
+{
+    const long long grid = std::max(1LL, std::min<long long>(4LL * c->num_cu,
+                                                             ((long long)A.total + 63) / 64));
+    hipLaunchKernelGGL((exact_kernel<NF, GRAD, DIM3, INT, RESUME>), dim3((unsigned)grid),
+                       dim3(64), 0, c->stream, P, A);
+    return hipGetLastError();
+}
+
+template <bool GRAD, bool DIM3, int INT, bool RESUME>
+static hipError_t dispatch_exact(rt_ctx *c, const Params &P, const TraceArgs &A)
+{
+    if (P.emission != kThSynch)
+        return hipSuccess;
+    if (P.nf == 1)
+        return launch_exact<1, GRAD, DIM3, INT, RESUME>(c, P, A);
+    if (P.nf <= 4)
+        return launch_exact<4, GRAD, DIM3, INT, RESUME>(c, P, A);
+    if (P.nf <= 8)
+        return launch_exact<8, GRAD, DIM3, INT, RESUME>(c, P, A);
+    return launch_exact<16, GRAD, DIM3, INT, RESUME>(c, P, A);
+}
+
+// Work items chunk by chunk (at most kMaxChunk each): with pct <= 0 all in
+// one persistent launch per chunk; otherwise the approach kernel over the
+// first pct percent of the chunk's work items, then the persistent kernel,
+// which resumes those and traces the rest from the camera (a chunk saves at
+// most A.st.n rays).  After each, the rays listed for the FP64 transfer.
 template <bool GRAD, bool DIM3, int INT>
 static hipError_t run_launches(rt_ctx *c, const Params &P, TraceArgs A, int total, int pct)
 {
     hipError_t e;
-    if (pct <= 0) {
-        A.w0 = 0;
-        A.total = total;
-        A.nsaved = 0;
-        hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue);
-        if ((e = hipGetLastError()) != hipSuccess)
-            return e;
-        return dispatch_nf<GRAD, DIM3, INT, false>(c, P, A);
-    }
     const long long chunk =
-        std::min<long long>(kMaxChunk, (long long)A.st.n * 100 / pct);   // saves <= st.n
+        pct <= 0 ? (long long)kMaxChunk
+                 : std::min<long long>(kMaxChunk, (long long)A.st.n * 100 / pct);   // saves <= st.n
+    // a one-chunk eager render times its kernels one by one
+    const bool mark = !c->capturing && total <= chunk;
+    if (!c->capturing)
+        c->last_chunks = (int)((total + chunk - 1) / chunk);
     for (long long w0 = 0; w0 < total; w0 += chunk) {
         A.w0 = (int)w0;
         A.total = (int)std::min<long long>(chunk, total - w0);
-        A.nsaved = (int)(((long long)A.total * pct + 99) / 100);
-        hipLaunchKernelGGL(approach_kernel<INT>, dim3((unsigned)((A.nsaved + 63) / 64)),
-                           dim3(64), 0, c->stream, P, A);
+        A.nsaved = pct <= 0 ? 0 : (int)(((long long)A.total * pct + 99) / 100);
+        if (A.nsaved > 0)
+            hipLaunchKernelGGL(approach_kernel<INT>, dim3((unsigned)((A.nsaved + 63) / 64)),
+                               dim3(64), 0, c->stream, P, A);
         hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue);
         if ((e = hipGetLastError()) != hipSuccess)
             return e;
-        if ((e = dispatch_nf<GRAD, DIM3, INT, true>(c, P, A)) != hipSuccess)
+        if (mark && (e = hipEventRecord(c->evA, c->stream)) != hipSuccess)
+            return e;
+        e = A.nsaved > 0 ? dispatch_nf<GRAD, DIM3, INT, true>(c, P, A)
+                         : dispatch_nf<GRAD, DIM3, INT, false>(c, P, A);
+        if (e != hipSuccess)
+            return e;
+        if (mark && (e = hipEventRecord(c->evT, c->stream)) != hipSuccess)
+            return e;
+        e = A.nsaved > 0 ? dispatch_exact<GRAD, DIM3, INT, true>(c, P, A)
+                         : dispatch_exact<GRAD, DIM3, INT, false>(c, P, A);
+        if (e != hipSuccess)
+            return e;
+        if (mark && (e = hipEventRecord(c->evX, c->stream)) != hipSuccess)
             return e;
     }
     return hipSuccess;
@@ -847,7 +982,8 @@ static int ensure_order(rt_ctx *c, int W, int H, int T, double sx, double sy, co
 }
 
 static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *freqs, int nf,
-                       double *d_image, double *d_dimage, rt_ray_counters *d_counters, bool grad)
+                       double *d_image, double *d_dimage, rt_ray_counters *d_counters,
+                       double *d_spectrum, bool grad)
 {
     if (!c)
         return RT_E_INVALID;
@@ -891,6 +1027,14 @@ static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *fr
     A.dimage = grad ? d_dimage : nullptr;
     A.counters = d_counters;
     A.st = RayState{};
+    if (!c->d_xlist) {
+        if (c->capturing)
+            return fail(c, RT_E_STATE, "the FP64 re-trace's ray list is allocated by an eager "
+                                       "render; render once before capturing");
+        HIPCHK(c, hipMalloc((void **)&c->d_xlist, (size_t)kMaxChunk * sizeof(int)));
+    }
+    A.xlist = c->d_xlist;
+    A.xcount = c->d_queue + 1;
     const int pct = c->cfg.approach != 0 ? c->cfg.approach
                                          : (total >= kApproachRays ? 100 : RT_APPROACH_NONE);
     if (pct > 0 &&
@@ -907,6 +1051,14 @@ static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *fr
         e = d3 ? dispatch_int<false, true>(c, P, A, total, pct)
                : dispatch_int<false, false>(c, P, A, total, pct);
     HIPCHK(c, e);
+    if (d_spectrum) {
+        const int nb = (int)std::min<long long>(kSpecBlocks, ((long long)P.npix + 1023) / 1024);
+        hipLaunchKernelGGL(spectrum_partial, dim3((unsigned)nb, (unsigned)ncam), dim3(256), 0,
+                           c->stream, (const double *)d_image, P.npix, nf, c->d_spec_part);
+        hipLaunchKernelGGL(spectrum_final, dim3((unsigned)ncam), dim3(64), 0, c->stream,
+                           (const double *)c->d_spec_part, nb, nf, d_spectrum);
+        HIPCHK(c, hipGetLastError());
+    }
     if (!c->capturing)
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     return RT_OK;
@@ -941,7 +1093,7 @@ static int render_host(rt_ctx *c, const double *incl, int ncam, const double *fr
     if (grad && (rc = ensure_buffer(c, &c->d_dimage, &c->dimage_cap, 2 * n * sizeof(double))))
         return rc;
     rc = render_impl(c, incl, ncam, freqs, nf, c->d_image, grad ? c->d_dimage : nullptr,
-                     nullptr, grad);
+                     nullptr, h_spectrum ? c->d_spec : nullptr, grad);
     if (rc)
         return rc;
     HIPCHK(c, hipMemcpyAsync(h_image, c->d_image, n * sizeof(double), hipMemcpyDeviceToHost,
@@ -949,21 +1101,13 @@ static int render_host(rt_ctx *c, const double *incl, int ncam, const double *fr
     if (grad)
         HIPCHK(c, hipMemcpyAsync(h_dimage, c->d_dimage, 2 * n * sizeof(double),
                                  hipMemcpyDeviceToHost, c->stream));
+    if (h_spectrum)
+        HIPCHK(c, hipMemcpyAsync(h_spectrum, c->d_spec, (size_t)ncam * nf * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
-    if (h_spectrum) {
-        // core.c:146-150: sequential sum over pixels
-        size_t npix = (size_t)c->params.width * c->params.height;
-        for (int k = 0; k < ncam; k++)
-            for (int f = 0; f < nf; f++) {
-                double s = 0.;
-                for (size_t i = 0; i < npix; i++)
-                    s += h_image[((size_t)k * npix + i) * nf + f];
-                h_spectrum[k * nf + f] = s;
-            }
-    }
     return RT_OK;
 }
 
@@ -994,12 +1138,22 @@ rt_ctx *rt_create(int device)
         (e = hipGetDeviceProperties(&prop, device)) != hipSuccess ||
         (e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess ||
-        (e = hipMalloc((void **)&c->d_queue, sizeof(unsigned int))) != hipSuccess) {
+        (e = hipEventCreate(&c->evA)) != hipSuccess || (e = hipEventCreate(&c->evT)) != hipSuccess ||
+        (e = hipEventCreate(&c->evX)) != hipSuccess ||
+        (e = hipMalloc((void **)&c->d_queue, 2 * sizeof(unsigned int))) != hipSuccess) {
         fail(nullptr, RT_E_HIP, "HIP set-up failed: %s", hipGetErrorString(e));
         rt_destroy(c);
         return nullptr;
     }
     c->num_cu = prop.multiProcessorCount;
+    if ((e = hipMalloc((void **)&c->d_spec_part,
+                       (size_t)kMaxCam * kSpecBlocks * kMaxFreq * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc((void **)&c->d_spec, (size_t)kMaxCam * kMaxFreq * sizeof(double))) !=
+            hipSuccess) {
+        fail(nullptr, RT_E_HIP, "HIP set-up failed: %s", hipGetErrorString(e));
+        rt_destroy(c);
+        return nullptr;
+    }
     std::vector<float2> tab;
     kappa_table(tab, c->ktab_u0, c->ktab_idu);
     c->ktab_n = (int)tab.size();
@@ -1023,6 +1177,9 @@ void rt_destroy(rt_ctx *c)
         (void)hipStreamSynchronize(c->stream);
     (void)hipFree(c->d_prims);
     (void)hipFree(c->d_queue);
+    (void)hipFree(c->d_xlist);
+    (void)hipFree(c->d_spec_part);
+    (void)hipFree(c->d_spec);
     (void)hipFree(c->d_ktab);
     for (auto &t : c->orders)
         (void)hipFree(t.d);
@@ -1034,6 +1191,9 @@ void rt_destroy(rt_ctx *c)
         (void)hipEventDestroy(c->ev0);
     if (c->ev1)
         (void)hipEventDestroy(c->ev1);
+    for (hipEvent_t ev : {c->evA, c->evT, c->evX})
+        if (ev)
+            (void)hipEventDestroy(ev);
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1123,11 +1283,12 @@ int rt_render_grad(rt_ctx *c, const double *incl, int ncam, const double *freqs,
 }
 
 int rt_render_device(rt_ctx *c, const double *incl, int ncam, const double *freqs, int nf,
-                     double *d_image, double *d_dimage, rt_ray_counters *d_counters)
+                     double *d_image, double *d_dimage, rt_ray_counters *d_counters,
+                     double *d_spectrum)
 {
     if (!c)
         return RT_E_INVALID;
-    return render_impl(c, incl, ncam, freqs, nf, d_image, d_dimage, d_counters,
+    return render_impl(c, incl, ncam, freqs, nf, d_image, d_dimage, d_counters, d_spectrum,
                        d_dimage != nullptr);
 }
 
@@ -1148,6 +1309,28 @@ int rt_synchronize(rt_ctx *c)
 }
 
 double rt_last_kernel_ms(const rt_ctx *c) { return c ? c->last_ms : 0.; }
+
+int rt_last_kernel_times(rt_ctx *c, double *out)
+{
+    if (!c || !out)
+        return RT_E_INVALID;
+    if (int rc0 = no_capture(c, "rt_last_kernel_times"))
+        return rc0;
+    if (c->last_chunks != 1)
+        return fail(c, RT_E_STATE, "the last render ran %d launch chunks; kernel times are "
+                                   "kept for renders of one chunk", c->last_chunks);
+    int rc = select_device(c);
+    if (rc)
+        return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const hipEvent_t ev[5] = {c->ev0, c->evA, c->evT, c->evX, c->ev1};
+    for (int i = 0; i < 4; i++) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+        out[i] = ms;
+    }
+    return RT_OK;
+}
 
 double rt_jansky_factor(const rt_ctx *c)
 {
@@ -1225,6 +1408,7 @@ int rt_graph_launch(rt_ctx *c, rt_graph *g)
     int rc = select_device(c);
     if (rc)
         return rc;
+    c->last_chunks = 0;   // no per-kernel times for a replay
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK(c, hipGraphLaunch(g->exec, c->stream));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));

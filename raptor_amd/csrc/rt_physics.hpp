@@ -506,11 +506,12 @@ __host__ __device__ inline void init_photon(const Camera &cm, double alpha, doub
 
 // Interpolated primitives at a point (grmhd.c:35-71, with the 3D extension).
 // The cell index and the weights come from the FP64 position; the weighted
-// sums are formed in FP32 from the FP32 primitives.
+// sums are formed in T (FP32 for the transfer pass, FP64 for the re-trace of
+// rays near the tau gate) from the FP32 primitives.
 // prims: 8 floats per cell, cells ordered ((i*n2 + j)*n3 + k).
-template <bool DIM3>
+template <bool DIM3, class T = float>
 __host__ __device__ inline void sample_prims(const Params &P, const float4 *__restrict__ prims,
-                                             const double y[8], float v[8])
+                                             const double y[8], T v[8])
 {
     // grmhd.c:46-71: i = (int)(t - 1/2) truncates toward zero, the clamps set
     // del to 0 / 1, and otherwise del = t - (i + 1/2) with t = (X - x0)/dx
@@ -537,9 +538,9 @@ __host__ __device__ inline void sample_prims(const Params &P, const float4 *__re
     } else {
         d2 = t2 - (j + 0.5);
     }
-    const float f1 = (float)d1, f2 = (float)d2;
-    const float c0 = (1.f - f1) * (1.f - f2), c1 = (1.f - f1) * f2;
-    const float c2 = f1 * (1.f - f2), c3 = f1 * f2;
+    const T f1 = (T)d1, f2 = (T)d2;
+    const T c0 = (T(1) - f1) * (T(1) - f2), c1 = (T(1) - f1) * f2;
+    const T c2 = f1 * (T(1) - f2), c3 = f1 * f2;
 #define RT_BI(f00, f01, f10, f11) ((f00) * c0 + (f01) * c1 + (f10) * c2 + (f11) * c3)
     if (!DIM3) {
         const size_t b00 = (size_t)(i * P.n2 + j) * 2;
@@ -570,7 +571,7 @@ __host__ __device__ inline void sample_prims(const Params &P, const float4 *__re
             ph -= P.period3;
         double t = ph * P.idx3 - 0.5;
         double fk = floor(t);
-        const float d3 = (float)(t - fk);
+        const T d3 = (T)(t - fk);
         // t lies in [-1/2, n3 - 1/2), so k in [-1, n3 - 1] (clamped: a NaN
         // position converts to an arbitrary integer on some targets)
         int k = (int)fk;
@@ -579,7 +580,7 @@ __host__ __device__ inline void sample_prims(const Params &P, const float4 *__re
         const size_t n3 = P.n3, n2 = P.n2;
         size_t b00 = ((size_t)i * n2 + j) * n3, b01 = b00 + n3;
         size_t b10 = b00 + n2 * n3, b11 = b10 + n3;
-        float w[2][8];
+        T w[2][8];
         const int ks[2] = {k0, k1};
 #pragma unroll
         for (int q = 0; q < 2; q++) {
@@ -598,7 +599,7 @@ __host__ __device__ inline void sample_prims(const Params &P, const float4 *__re
             w[q][7] = RT_BI(p00b.w, p01b.w, p10b.w, p11b.w);
         }
         for (int q = 0; q < 8; q++)
-            v[q] = (1.f - d3) * w[0][q] + d3 * w[1][q];
+            v[q] = (T(1) - d3) * w[0][q] + d3 * w[1][q];
     }
 #undef RT_BI
 }
@@ -947,10 +948,12 @@ template <int NF, bool GRAD>
 struct Accum {
     float tau[NF], ctau[NF];
     float I[NF];
+    float wrel;   // sum over samples in the gate window of dI / I (see kGateMargin)
     double dtau[GRAD ? NF : 1][2];
     double dI[GRAD ? NF : 1][2];
     __host__ __device__ void clear()
     {
+        wrel = 0.f;
         for (int f = 0; f < NF; f++) {
             tau[f] = ctau[f] = 0.f;
             I[f] = 0.f;
@@ -961,6 +964,40 @@ struct Accum {
         }
     }
 };
+
+// The tau gate (radiative_transfer.c:90-95) is a discontinuity: a sample adds
+// its emission only while tau < ln 1000.  The FP32 tau differs from an FP64
+// one by up to ~3e-4 where it reaches the gate (measured over a 1024^2 image:
+// the FP32 fluid sample's rounding, accumulated to tau ~ 7), so at samples
+// whose FP32 tau lies within kGateMargin of ln 1000 the two may decide
+// differently.  Each such sample adds its dI / I to Accum::wrel (I after the
+// sample, or as if it had been added); a ray whose wrel exceeds kGateFlip is
+// traced again with the FP64 transfer (transfer_sample_f64), so that no
+// pixel can differ from an FP64 transfer by more than kGateFlip through a
+// decision at the gate.  About 1 ray in 3000 at 1024^2 and 230 GHz.
+constexpr float kGateMargin = 1e-3f;
+constexpr float kGateFlip = 5e-4f;
+
+// The FP64 transfer's optical depth lives in the FP32 pair (tau, ctau) of its
+// accumulator, which it does not otherwise use (bit pattern of one double).
+__host__ __device__ inline double tau_f64(const float &lo, const float &hi)
+{
+    uint32_t a, b;
+    memcpy(&a, &lo, 4);
+    memcpy(&b, &hi, 4);
+    const uint64_t u = ((uint64_t)b << 32) | a;
+    double d;
+    memcpy(&d, &u, 8);
+    return d;
+}
+__host__ __device__ inline void set_tau_f64(float &lo, float &hi, double d)
+{
+    uint64_t u;
+    memcpy(&u, &d, 8);
+    const uint32_t a = (uint32_t)u, b = (uint32_t)(u >> 32);
+    memcpy(&lo, &a, 4);
+    memcpy(&hi, &b, 4);
+}
 
 __host__ __device__ inline void kahan_add(float &s, float &c, float x)
 {
@@ -1048,7 +1085,156 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
                 acc.dI[f][1] += (double)dI * (dlnj1 - acc.dtau[f][1]);
             }
         }
+        if (fabsf(tau - lnth) < kGateMargin && dI > 0.f) {
+            // I with this sample added either way; 0 (a first sample at the
+            // gate) gives an infinite share, which flags the ray
+            const float Iw = tau < lnth ? acc.I[f] : acc.I[f] + dI;
+            acc.wrel += dI * frcp(Iw);
+        }
     }
+}
+
+// ---- FP64 transfer for rays flagged at the tau gate -------------------------
+// get_fluid_params (raptor_harm_model.c:190-320), pitch_angle and
+// freq_in_plasma_frame (radiative_transfer.c:17-47) in FP64, in the
+// reference's form (its electron temperature ratio, metric and lowering),
+// from the same FP32 primitives interpolated in FP64.
+struct Fluid64 {
+    double ne, thetae, B, mu, gdop, dthetae;
+};
+
+template <bool DIM3>
+__host__ __device__ inline bool fluid_sample_f64(const Params &P, const float4 *__restrict__ prims,
+                                                 const Point &pt, const double y[8], Fluid64 &F)
+{
+    double v[8];
+    sample_prims<DIM3, double>(P, prims, y, v);
+    const double rho = v[0], uu = v[1];
+    const MetricDD g = metric_dd(pt, P.a, P.hslope);
+    const double r = pt.r;
+    const double rho2 = r * r + P.a * P.a * (pt.c * pt.c);
+    const double gcon00 = -1. - 2. * r / rho2, gcon01 = 2. / rho2;   // metric.c:207-231
+    const double V1 = v[2], V2 = v[3], V3 = v[4];
+    const double VdotV = g.g11 * V1 * V1 + 2. * g.g13 * V1 * V3 + g.g22 * V2 * V2 + g.g33 * V3 * V3;
+    const double Vfac = sqrt(-1. / gcon00 * (1. + fabs(VdotV)));
+    const double u[4] = {-Vfac * gcon00, V1 - Vfac * gcon01, V2, V3};
+    double ud[4];
+    lower(g, u, ud);
+    const double UdotBp = ud[1] * v[5] + ud[2] * v[6] + ud[3] * v[7];
+    const double b[4] = {UdotBp, (v[5] + u[1] * UdotBp) / u[0], (v[6] + u[2] * UdotBp) / u[0],
+                         (v[7] + u[3] * UdotBp) / u[0]};
+    double bd[4];
+    lower(g, b, bd);
+    const double Bsq = b[0] * bd[0] + b[1] * bd[1] + b[2] * bd[2] + b[3] * bd[3];
+    F.ne = rho * P.Ne_unit;
+    F.B = sqrt(Bsq) * P.B_unit + 1e-20;
+    const double beta = uu * (4. / 3. - 1.) / (0.5 * (Bsq + 1e-20));
+    const double b2 = beta * beta;
+    const double trat = P.R_high * b2 / (1. + b2) + P.R_low / (1. + b2);
+    const double gam2 = 0.5 * ((1. + 2. / 3. * (trat + 1.) / (trat + 2.)) + 4. / 3.);
+    const double mr = kProtonMass / kElectronMass;
+    F.thetae = (uu / rho) * ((gam2 - 1.) * mr / (1. + trat));
+    const double dgam2 = (1. / 3.) / ((trat + 2.) * (trat + 2.));
+    F.dthetae = (uu / rho) * (mr * (dgam2 * (1. + trat) - (gam2 - 1.)) /
+                              ((1. + trat) * (1. + trat))) * (b2 / (1. + b2));
+    if (F.thetae < 0.) {
+        F.thetae = 1e-20;
+        F.dthetae = 0.;
+    }
+    if (Bsq / rho > 1.)
+        return false;
+    double kd[4];
+    lower(g, y + 4, kd);
+    const double ku = kd[0] * u[0] + kd[1] * u[1] + kd[2] * u[2] + kd[3] * u[3];
+    double mu = 0.;
+    if (F.B != 0.) {
+        mu = (kd[0] * b[0] + kd[1] * b[1] + kd[2] * b[2] + kd[3] * b[3]) /
+             (fabs(ku) * (F.B / P.B_unit));
+        if (fabs(mu) > 1.)
+            mu /= fabs(mu);
+    }
+    F.mu = mu;
+    F.gdop = -ku;
+    return true;
+}
+
+// radiative_transfer.c:71-98 for one sample in FP64 with the thermal fit
+// (j_nu.c:130-153) and Kirchhoff's absorptivity with the Planck function
+// (j_nu.c:188-200) as the reference writes it, exp(x) - 1: at radio
+// frequencies x = h nu / k T_e is ~1e-9 and that form is off by ~1e-16 / x
+// relative, which this path reproduces, since it exists to reach the gate
+// decisions of the reference's FP64 optical depth.  tau lives in the
+// accumulator's (tau, ctau) pair; I and, with GRAD, the tangents accumulate
+// as in transfer_sample.  Returns false where the sigma cut skips the sample.
+template <int NF, bool GRAD, bool DIM3>
+__host__ __device__ inline bool transfer_sample_f64(const Params &P,
+                                                    const float4 *__restrict__ prims,
+                                                    const Point &pt, const double y[8],
+                                                    double dlabs, Accum<NF, GRAD> &acc)
+{
+    Fluid64 F;
+    if (!fluid_sample_f64<DIM3>(P, prims, pt, y, F))
+        return false;
+    const double lnth = 6.907755278982137;   // log(1000.)
+    const double dl = dlabs * P.Rg;
+    const double sth = sqrt(1. - F.mu * F.mu);
+    const double nu_c = kElectronCharge * F.B / (2. * kPi * kElectronMass * kC);
+    const double nu_s = (2. / 9.) * nu_c * F.thetae * F.thetae * sth;
+    const bool on = !(fabs(sth) < 1.e-10) && F.thetae > 1e-4;
+    const double xk = kPlanck / (kElectronMass * kC * kC * F.thetae);   // x = h nu_p / k T_e
+    const double c1112 = 1.8877486253633868;                            // 2^(11/12)
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        // unrolled (the accumulators stay in registers); frequencies past
+        // P.nf are skipped
+        if (f >= P.nf)
+            continue;
+        const double nu = P.freq[f];
+        double nu_p = F.gdop * nu;
+        if (nu_p <= 0.)
+            nu_p = 1.;
+        const double x = xk * nu_p;
+        double j = 0., s6 = 0.;
+        if (on && nu_p != 1.) {
+            const double X = nu_p / nu_s;
+            s6 = exp(log(X) * (1. / 6.));          // X^(1/6)
+            const double ff = s6 * s6 * s6 + c1112 * s6;
+            j = F.ne * (1.4142135623730951 * kPi / 3.) * nu_s / (2. * F.thetae * F.thetae) *
+                (ff * ff) * exp(-(s6 * s6)) * kE2C;
+        }
+        // alpha = j (e^2/ch) / B_nu / (e^2/c), B_nu = 2 nu^3 / c^2 / (e^x - 1): 0 for
+        // j = 0 unless B_nu is 0 (e^x overflows), where the reference's 0/0 is NaN
+        const double Bnu = 2. * nu_p * nu_p * nu_p / (kC * kC) * (1. / (exp(x) - 1.));
+        const double alpha = j != 0. ? j * kE2CH / Bnu / kE2C : (Bnu == 0. ? NAN : 0.);
+        const double dtau = nu_p / nu * alpha * dl;
+        double tau = tau_f64(acc.tau[f], acc.ctau[f]) + dtau;
+        set_tau_f64(acc.tau[f], acc.ctau[f], tau);
+        const double gi = nu / nu_p;
+        const double dI = j * exp(-tau) * dl * (gi * gi);
+        double dlnj0 = 0., dlnj1 = 0., dlna1 = 0.;
+        if (GRAD && j != 0.) {
+            const double cb = s6 * s6, sx = cb * s6;
+            const double Dx = 2. * (0.5 * sx + c1112 / 6. * s6) / (sx + c1112 * s6) - cb / 3.;
+            const double dlnB = 0.5 * (F.B - 1e-20) / F.B;
+            const double dlnT = F.dthetae / F.thetae;
+            const double dlnBnu = x / (-expm1(-x));
+            dlnj0 = 1. + dlnB * (1. - Dx);
+            dlnj1 = -2. * Dx * dlnT;
+            dlna1 = dlnj1 - dlnBnu * dlnT;
+        }
+        if (GRAD && dtau != 0.) {
+            acc.dtau[f][0] += dtau * dlnj0;
+            acc.dtau[f][1] += dtau * dlna1;
+        }
+        if (tau < lnth && !(tau < 0.)) {
+            acc.I[f] += (float)dI;
+            if (GRAD && dI != 0.) {
+                acc.dI[f][0] += dI * (dlnj0 - acc.dtau[f][0]);
+                acc.dI[f][1] += dI * (dlnj1 - acc.dtau[f][1]);
+            }
+        }
+    }
+    return true;
 }
 
 // A ray being traced.
@@ -1062,6 +1248,16 @@ struct Ray {
     bool terminate;
     Accum<NF, GRAD> acc;
 };
+
+// Whether a ray traced with the FP32 transfer is to be traced again with the
+// FP64 one: its FP32 tau passed the gate within kGateMargin at samples
+// carrying more than kGateFlip of its intensity (the thermal fit, whose FP64
+// form is above).
+template <int EMIS, int NF, bool GRAD>
+__host__ __device__ inline bool needs_exact(const Ray<NF, GRAD> &R)
+{
+    return EMIS == kThSynch && R.acc.wrel > kGateFlip;
+}
 
 __host__ __device__ inline void pixel_impact(const Params &P, int pix, double &alpha,
                                              double &beta)
@@ -1124,16 +1320,23 @@ __host__ __device__ inline bool in_grid(const Params &P, const double y[8])
 }
 
 // The rest of the iteration (integrator.c:277-284): the sample of the new
-// point, when it is inside the grid, with the step's |dlambda|.
-template <int NF, bool GRAD, bool DIM3, int EMIS = kThSynch>
+// point, when it is inside the grid, with the step's |dlambda|; EXACT: with
+// the FP64 transfer (thermal fit) instead of the FP32 one.
+template <int NF, bool GRAD, bool DIM3, int EMIS = kThSynch, bool EXACT = false>
 __host__ __device__ inline void ray_sample(const Params &P, const float4 *__restrict__ prims,
                                            Ray<NF, GRAD> &R, double dl)
 {
     if (in_grid(P, R.y)) {
-        Fluid F;
-        if (fluid_sample<DIM3>(P, prims, R.pt, R.y, F)) {
-            R.samples++;
-            transfer_sample<NF, GRAD, EMIS>(P, F, fabs(dl), R.acc);
+        if constexpr (EXACT) {
+            static_assert(EMIS == kThSynch, "the FP64 transfer is the thermal fit's");
+            if (transfer_sample_f64<NF, GRAD, DIM3>(P, prims, R.pt, R.y, fabs(dl), R.acc))
+                R.samples++;
+        } else {
+            Fluid F;
+            if (fluid_sample<DIM3>(P, prims, R.pt, R.y, F)) {
+                R.samples++;
+                transfer_sample<NF, GRAD, EMIS>(P, F, fabs(dl), R.acc);
+            }
         }
     }
     R.steps++;
@@ -1141,14 +1344,14 @@ __host__ __device__ inline void ray_sample(const Params &P, const float4 *__rest
 
 // One iteration of integrate_geodesic's loop (integrator.c:243-284).
 // Returns false when the ray escaped through the outer grid edge.
-template <int NF, bool GRAD, bool DIM3, int INT = kRK2, int EMIS = kThSynch>
+template <int NF, bool GRAD, bool DIM3, int INT = kRK2, int EMIS = kThSynch, bool EXACT = false>
 __host__ __device__ inline bool ray_step(const Params &P, const float4 *__restrict__ prims,
                                          Ray<NF, GRAD> &R)
 {
     double dl;
     if (!ray_advance<INT>(P, R, dl))
         return false;
-    ray_sample<NF, GRAD, DIM3, EMIS>(P, prims, R, dl);
+    ray_sample<NF, GRAD, DIM3, EMIS, EXACT>(P, prims, R, dl);
     return true;
 }
 

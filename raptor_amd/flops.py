@@ -57,6 +57,25 @@ def image_flops_split(steps, samples, n_freq, grad=False):
     return f64, f32
 
 
+def kernel_split(steps, grid_step, samples, n_freq, grad=False):
+    """Algorithmic work of the split launch's two kernels, given per-ray
+    steps, grid_step (steps taken before the first one that lands inside the
+    grid, -1 if none does) and samples: ((FP64, FP32) of the approach
+    kernel, (FP64, FP32) of the trace kernel).  The approach kernel takes a
+    ray up to its grid entry (a ray that never reaches the grid, all its
+    steps); the trace kernel repeats the entering step and takes every step
+    and transfer sample after it.  The approach kernel's evaluation of the
+    entering step, which the trace kernel repeats, is not algorithmic work."""
+    import numpy as np
+    steps = np.asarray(steps, dtype=np.float64)
+    gs = np.asarray(grid_step, dtype=np.float64)
+    enter = gs >= 0
+    ns_app = float(np.sum(np.where(enter, gs, steps)))
+    ns_tr = float(np.sum(np.where(enter, steps - gs, 0.0)))
+    _, f32 = image_flops_split(0, samples, n_freq, grad=grad)
+    return (ns_app * F_STEP, 0.0), (ns_tr * F_STEP, f32)
+
+
 def mix_peak(f64, f32, peak64, peak32):
     """Peak rate for a mix of FP64 and FP32 work: total work over the time it
     takes when each part runs at its own peak."""

@@ -79,9 +79,11 @@ SIGNATURES = [
     ("rt_load_prims", ctypes.c_int, [_vp, ctypes.POINTER(RtGrid), ctypes.POINTER(ctypes.c_float)]),
     ("rt_render", ctypes.c_int, [_vp, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]),
     ("rt_render_grad", ctypes.c_int, [_vp, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp]),
-    ("rt_render_device", ctypes.c_int, [_vp, _dp, ctypes.c_int, _dp, ctypes.c_int, _vp, _vp, _vp]),
+    ("rt_render_device", ctypes.c_int, [_vp, _dp, ctypes.c_int, _dp, ctypes.c_int, _vp, _vp, _vp,
+                                         _vp]),
     ("rt_synchronize", ctypes.c_int, [_vp]),
     ("rt_last_kernel_ms", ctypes.c_double, [_vp]),
+    ("rt_last_kernel_times", ctypes.c_int, [_vp, _dp]),
     ("rt_jansky_factor", ctypes.c_double, [_vp]),
     ("rt_set_launch_config", ctypes.c_int, [_vp, ctypes.POINTER(RtLaunchConfig)]),
     ("rt_get_launch_config", ctypes.c_int, [_vp, ctypes.POINTER(RtLaunchConfig)]),
@@ -227,17 +229,28 @@ class Tracer:
                                        _d(img), _d(spec)))
         return img, spec
 
-    def render_device(self, inclinations, freqs, d_image, d_dimage=None, d_counters=None):
+    def render_device(self, inclinations, freqs, d_image, d_dimage=None, d_counters=None,
+                      d_spectrum=None):
+        """The pixel loop into device buffers (addresses from alloc): image
+        [ncam, H*W, nf], optional dimage [ncam, H*W, nf, 2], per-ray counters
+        [ncam*H*W] and spectrum [ncam, nf]; returns without synchronising."""
         incl = np.ascontiguousarray(np.atleast_1d(inclinations), float)
         fr = np.ascontiguousarray(np.atleast_1d(freqs), float)
         self._check(self.lib.rt_render_device(self.ctx, _d(incl), incl.size, _d(fr), fr.size,
-                                              d_image, d_dimage, d_counters))
+                                              d_image, d_dimage, d_counters, d_spectrum))
 
     def synchronize(self):
         self._check(self.lib.rt_synchronize(self.ctx))
 
     def last_kernel_ms(self):
         return self.lib.rt_last_kernel_ms(self.ctx)
+
+    def last_kernel_times(self):
+        """ms of the last render's approach, trace, FP64 re-trace and spectrum
+        kernels (renders of one launch chunk)."""
+        out = np.zeros(4)
+        self._check(self.lib.rt_last_kernel_times(self.ctx, _d(out)))
+        return dict(zip(("approach", "trace", "exact", "spectrum"), out))
 
     def jansky_factor(self):
         return self.lib.rt_jansky_factor(self.ctx)

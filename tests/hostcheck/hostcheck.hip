@@ -26,16 +26,46 @@ static inline __host__ void sincospi(double x, double *s, double *c)
 
 using namespace rt;
 
-template <int NF, bool GRAD, int INT, int EMIS>
-static void finish(const Params &P, const float4 *prims, int ray, Ray<NF, GRAD> &R, double *img,
-                   double *dimg, int *cnt)
+// Traces R from its start state to the end; a ray that needs_exact is started
+// again by `restart` and traced with the FP64 transfer, as the kernel does.
+template <int NF, bool GRAD, int INT, int EMIS, bool EXACT, class Restart>
+static void trace_to_end(const Params &P, const float4 *prims, Ray<NF, GRAD> &R,
+                         Restart &&restart, bool retrace = true)
 {
     const bool dim3 = P.n3 > 1;
     while (ray_alive<NF, GRAD>(P, R)) {
-        bool cont = dim3 ? ray_step<NF, GRAD, true, INT, EMIS>(P, prims, R)
-                         : ray_step<NF, GRAD, false, INT, EMIS>(P, prims, R);
+        bool cont = dim3 ? ray_step<NF, GRAD, true, INT, EMIS, EXACT>(P, prims, R)
+                         : ray_step<NF, GRAD, false, INT, EMIS, EXACT>(P, prims, R);
         if (!cont)
             break;
+    }
+    if constexpr (!EXACT && EMIS == kThSynch) {
+        if (retrace && needs_exact<EMIS>(R)) {
+            restart(R);
+            trace_to_end<NF, GRAD, INT, EMIS, true>(P, prims, R, restart);
+        }
+    }
+}
+
+// Transfer of the thermal fit, for tests (hc_render_mode): 0 as the kernels
+// (FP32, rays at the tau gate again in FP64), 1 every ray with the FP64
+// transfer, 2 FP32 only (no second trace).
+static int g_mode = 0;
+
+template <int NF, bool GRAD, int INT, int EMIS, class Restart>
+static void finish(const Params &P, const float4 *prims, int ray, Ray<NF, GRAD> &R, double *img,
+                   double *dimg, int *cnt, Restart &&restart)
+{
+    if constexpr (EMIS == kThSynch) {
+        if (g_mode == 1)
+            trace_to_end<NF, GRAD, INT, EMIS, true>(P, prims, R, restart);
+        else if (g_mode == 2)
+            trace_to_end<NF, GRAD, INT, EMIS, false>(P, prims, R, [](Ray<NF, GRAD> &) {},
+                                                     false);
+        else
+            trace_to_end<NF, GRAD, INT, EMIS, false>(P, prims, R, restart);
+    } else {
+        trace_to_end<NF, GRAD, INT, EMIS, false>(P, prims, R, restart);
     }
     for (int f = 0; f < P.nf; f++) {
         img[(size_t)ray * P.nf + f] = R.acc.I[f];
@@ -61,8 +91,9 @@ static void run(const Params &P, const float4 *prims, int total, int split, doub
     for (int ray = 0; ray < total; ray++) {
         Ray<NF, GRAD> R;
         if (!split) {
-            ray_init<NF, GRAD>(P, ray / P.npix, ray % P.npix, R);
-            finish<NF, GRAD, INT, EMIS>(P, prims, ray, R, img, dimg, cnt);
+            auto init = [&](Ray<NF, GRAD> &Q) { ray_init<NF, GRAD>(P, ray / P.npix, ray % P.npix, Q); };
+            init(R);
+            finish<NF, GRAD, INT, EMIS>(P, prims, ray, R, img, dimg, cnt, init);
             continue;
         }
         Ray<1, false> A, S;
@@ -80,17 +111,20 @@ static void run(const Params &P, const float4 *prims, int total, int split, doub
             }
             continue;
         }
-        for (int q = 0; q < 8; q++)
-            R.y[q] = S.y[q];
-        R.pt = S.pt;
-        R.steps = S.steps;
-        R.turns = S.turns;
-        R.thetadot_sign = S.thetadot_sign;
-        R.beta_sign = S.beta_sign;
-        R.samples = 0;
-        R.terminate = false;
-        R.acc.clear();
-        finish<NF, GRAD, INT, EMIS>(P, prims, ray, R, img, dimg, cnt);
+        auto load = [&](Ray<NF, GRAD> &Q) {
+            for (int q = 0; q < 8; q++)
+                Q.y[q] = S.y[q];
+            Q.pt = S.pt;
+            Q.steps = S.steps;
+            Q.turns = S.turns;
+            Q.thetadot_sign = S.thetadot_sign;
+            Q.beta_sign = S.beta_sign;
+            Q.samples = 0;
+            Q.terminate = false;
+            Q.acc.clear();
+        };
+        load(R);
+        finish<NF, GRAD, INT, EMIS>(P, prims, ray, R, img, dimg, cnt, load);
     }
 }
 
@@ -177,6 +211,19 @@ extern "C" int hc_render_split(const rt_params *p, const rt_grid *g, const float
         go(std::integral_constant<int, kRK2>());
     delete P;
     return 0;
+}
+
+// The image with the thermal fit's transfer in the given mode (g_mode).
+extern "C" int hc_render_mode(const rt_params *p, const rt_grid *g, const float *soa,
+                              const double *incl, int ncam, const double *freqs, int nf,
+                              int mode, double *img, double *dimg, int *cnt)
+{
+    if (p->emission != RT_EMIS_THSYNCH || mode < 0 || mode > 2)
+        return -1;
+    g_mode = mode;
+    const int rc = hc_render_split(p, g, soa, incl, ncam, freqs, nf, 0, 0., img, dimg, cnt);
+    g_mode = 0;
+    return rc;
 }
 
 // The midpoint trigonometry from the start point (shifted_point) next to a

@@ -15,8 +15,11 @@ pytestmark = pytest.mark.gpu
 
 FLUX_TOL = 1e-4
 PIX_TOL = 1e-3
-# distance of the FP64 tau from ln 1000 at the gate crossing below which the
-# FP32 tau may decide the gate differently (its error is ~1e-6)
+# distance of the FP64 tau from ln 1000 at the gate crossing below which an
+# FP32 tau (error up to ~3e-4 there) may decide the gate differently.  The
+# thermal fit's rays at the gate are traced again with the FP64 transfer, so
+# its images meet the bar on every pixel; the other emission models (no FP64
+# form) still hold these pixels apart.
 GATE_EPS = 1e-4
 
 
@@ -97,7 +100,7 @@ def test_maximum_batch_cameras_and_frequencies(tracer, dump128):
                 assert np.all(img[c, :, f] == 0)
                 continue
             r = rel_err(img[c, :, f], ora[:, f])
-            worst = max(worst, r[~(gate[:, f] < GATE_EPS)].max())
+            worst = max(worst, r.max())
             assert spec[c, f] == pytest.approx(img[c, :, f].sum(), rel=1e-13)
     assert worst < PIX_TOL
     with pytest.raises(rt.RtError, match="n_cam"):
@@ -151,8 +154,7 @@ def test_gradients_several_frequencies_match_oracle(tracer, dump128, nf):
     img, dimg, _ = tracer.render([INCL], freqs, grad=True)
     oimg, odimg, gate = O.render(P, dump128.prims, freqs, grad=True, gate=True)
     for f in range(nf):
-        off = ~(gate[:, f] < GATE_EPS)
-        assert rel_err(img[0, :, f], oimg[:, f])[off].max() < PIX_TOL, f
+        assert rel_err(img[0, :, f], oimg[:, f]).max() < PIX_TOL, f
         ipix = np.maximum(oimg[:, f], 1e-6 * oimg[:, f].max())
         for q, p in enumerate((1.0, R_HIGH)):   # d/d ln M_unit, R_high d/d R_high
             g, o = dimg[0, :, f, q], odimg[:, f, q]
@@ -234,17 +236,17 @@ def test_3d_dump_several_frequencies_match_oracle(tracer, dump3d_small, nf):
                 continue
             r = _report("3d i=%g nu=%.3g" % (inc, freqs[f]), img[c, :, f], ora[:, f])
             assert abs(img[c, :, f].sum() / ora[:, f].sum() - 1) < FLUX_TOL
-            assert r[~(gate[:, f] < GATE_EPS)].max() < PIX_TOL
+            assert r.max() < PIX_TOL
             assert spec[c, f] == pytest.approx(img[c, :, f].sum(), rel=1e-13)
 
 
 def test_c1_full_image(tracer, dump256):
-    """The 1024^2 image of the 256^2 dump against the oracle on every pixel.
-    Pixels whose FP64 tau passes within GATE_EPS of the ln 1000 gate
-    (radiative_transfer.c:90-95) at a sample are held apart: there an FP32
-    tau (error ~1e-6) may fall on the other side of the gate and add or drop
-    one sample's exp(-6.9)-weighted term.  They must be rare; every other
-    pixel meets the per-pixel bar."""
+    """The 1024^2 image of the 256^2 dump against the oracle on every pixel,
+    with no pixel held apart: rays whose FP32 optical depth passes the
+    ln 1000 gate (radiative_transfer.c:90-95) within the margin where an FP32
+    and an FP64 tau may decide it differently, at samples that would show in
+    the pixel, are traced again with the FP64 transfer.  The gate-adjacent
+    pixels (FP64 tau within GATE_EPS of ln 1000 at a sample) are reported."""
     W = 1024
     P = _setup(tracer, dump256, W)
     img, spec = tracer.render([INCL], [FREQ])
@@ -258,10 +260,10 @@ def test_c1_full_image(tracer, dump256):
           "elsewhere max rel %.3g" % (np.isfinite(gate).sum(), near.sum(), GATE_EPS,
                                       r[near].max() if near.any() else 0., r[~near].max()))
     assert abs(gpu.sum() / ora.sum() - 1) < FLUX_TOL
-    assert r[~near].max() < PIX_TOL
-    assert near.sum() < 1e-4 * W * W
+    assert r.max() < PIX_TOL
     img2, spec2 = tracer.render([INCL], [FREQ])
     assert spec2[0, 0] == spec[0, 0]
+    assert np.array_equal(img2, img)
 
 
 @pytest.mark.parametrize("case", sorted(SPACETIME_CASES))
@@ -290,7 +292,7 @@ def test_other_spacetimes_and_step_sizes(tracer, case):
     assert ora.max() > 1e-12
     r = _report(case, gpu, ora[:, 0])
     assert abs(gpu.sum() / ora[:, 0].sum() - 1) < FLUX_TOL
-    assert r[~(gate[:, 0] < GATE_EPS)].max() < PIX_TOL
+    assert r.max() < PIX_TOL
     assert (cnt[:, 0] == ocnt[:, 0]).mean() > 0.99 and (cnt[:, 1] == ocnt[:, 2]).mean() > 0.99
 
 
@@ -316,7 +318,7 @@ def test_other_integrators(tracer, dump128, integrator):
     ora, ocnt, gate = O.render(P, dump128.prims, [FREQ], counters=True, gate=True)
     r = _report("integrator %d" % integrator, gpu, ora[:, 0])
     assert abs(gpu.sum() / ora[:, 0].sum() - 1) < FLUX_TOL
-    assert r[~(gate[:, 0] < GATE_EPS)].max() < PIX_TOL
+    assert r.max() < PIX_TOL
     assert (cnt[:, 0] == ocnt[:, 0]).mean() > 0.99 and (cnt[:, 1] == ocnt[:, 2]).mean() > 0.99
 
 
