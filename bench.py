@@ -247,7 +247,12 @@ def bench_c1(args):
                       "%.1f s wall" % (c1 - c0)}
         result["speedup_vs_cpu"] = value / cpu_rate
         result["parity"] = _parity(img[pix], ora[:, 0], gate[:, 0])
-        result["parity"]["steps_equal_oracle_frac"] = float(np.mean(cnt[pix, 0] == ocnt[:, 0]))
+        # a ray whose tau passed ln 1000 ends there (nothing later reaches its
+        # pixel): its steps lie after the oracle's opaque step, within its trace
+        st, op = cnt[pix, 0], ocnt[:, 5]
+        agree = np.where(op >= 0, (st > op) & (st <= ocnt[:, 0]), st == ocnt[:, 0])
+        result["parity"]["steps_agree_oracle_frac"] = float(np.mean(agree))
+        result["parity"]["opaque_frac"] = float(np.mean(op >= 0))
         grid_step = (ocnt[:, 4], ocnt[:, 0])
 
     roof = _rooflines(cnt, 1, kms, per, grid_step, kernel=trace_name) if thermal else None

@@ -192,9 +192,10 @@ def ray_path(P, alpha, beta, maxpts=200000):
 
 def render(P, prims, freqs, pixels=None, grad=False, counters=False, gate=False, nthreads=0):
     """Trace pixels (all by default).  Returns image [npix, nf] (I_nu, cgs),
-    optionally dimage [npix, nf, 2], counters [npix, 5] (steps, samples in the
+    optionally dimage [npix, nf, 2], counters [npix, 6] (steps, samples in the
     grid, samples past the sigma cut, escaped, steps before the first one that
-    lands inside the grid or -1) and gate [npix, nf]: the
+    lands inside the grid or -1, steps taken when every frequency's tau had
+    passed ln 1000 or -1) and gate [npix, nf]: the
     distance of tau from ln 1000 at the sample where it crosses that gate
     (inf if it never does)."""
     p = np.ascontiguousarray(prims, float)
@@ -209,7 +210,7 @@ def render(P, prims, freqs, pixels=None, grad=False, counters=False, gate=False,
         pix_ptr = pixels.ctypes.data_as(_ip)
     img = np.zeros((npix, nf))
     dimg = np.zeros((npix, nf, 2)) if grad else None
-    cnt = np.zeros((npix, 5), dtype=np.int32) if counters else None
+    cnt = np.zeros((npix, 6), dtype=np.int32) if counters else None
     gt = np.zeros((npix, nf)) if gate else None
     rc = lib().or_render(ctypes.byref(P), _d(p), nf, _d(freqs), pix_ptr, npix, _d(img),
                          _d(dimg) if grad else None,

@@ -85,6 +85,8 @@ typedef struct {
 typedef struct {
     int steps, n_grid, n_rt, escaped;
     int grid_step;   /* steps taken before the first one that lands inside the grid, or -1 */
+    int opaque_step; /* steps taken when every frequency's tau had passed ln 1000, or -1:
+                      * after it no sample changes I or its tangents */
     double gate[64];
 } or_counters;
 
@@ -790,7 +792,7 @@ void or_trace_ray(const or_params *P, const double *prims, int x, int yy, int nf
     double r_current = exp(y[1]);
     int steps = 0, TERMINATE = 0, theta_turns = 0;
     double thetadot_prev = 0.;
-    int n_grid = 0, n_rt = 0, escaped = 0, grid_step = -1;
+    int n_grid = 0, n_rt = 0, escaped = 0, grid_step = -1, opaque_step = -1;
     const double lnth = log(1000.);
     double gate[64];
     for (int f = 0; f < nf; f++)
@@ -859,6 +861,13 @@ void or_trace_ray(const or_params *P, const double *prims, int x, int yy, int nf
                             dI[2 * f + q] += (dIk != 0.) ? dIk * (dlnj[q] - dtau_t[f][q]) : 0.;
                     }
                 }
+                if (opaque_step < 0) {
+                    int f = 0;
+                    while (f < nf && tau[f] >= lnth)
+                        f++;
+                    if (f == nf)
+                        opaque_step = steps;
+                }
             }
         }
         steps = steps + 1;
@@ -869,6 +878,7 @@ void or_trace_ray(const or_params *P, const double *prims, int x, int yy, int nf
         cnt->n_rt = n_rt;
         cnt->escaped = escaped;
         cnt->grid_step = grid_step;
+        cnt->opaque_step = opaque_step;
         for (int f = 0; f < nf; f++)
             cnt->gate[f] = gate[f];
     }
@@ -876,7 +886,7 @@ void or_trace_ray(const or_params *P, const double *prims, int x, int yy, int nf
 
 /* core.c:74-152 calculate_image over a pixel list.
  * image[p*nf + f] = I_nu (after the nu^3 scaling of core.c:136) for pixel
- * pix[p] (index y*W + x); dimage (optional) [p][f][2]; counters optional [p][5];
+ * pix[p] (index y*W + x); dimage (optional) [p][f][2]; counters optional [p][6];
  * gate (optional) [p][f]: the tau-gate margin of or_counters.
  * nthreads <= 0 uses the OpenMP default. */
 int or_render(const or_params *P, const double *prims, int nf, const double *freqs,
@@ -909,11 +919,12 @@ int or_render(const or_params *P, const double *prims, int nf, const double *fre
             }
         }
         if (counters) {
-            counters[p * 5 + 0] = c.steps;
-            counters[p * 5 + 1] = c.n_grid;
-            counters[p * 5 + 2] = c.n_rt;
-            counters[p * 5 + 3] = c.escaped;
-            counters[p * 5 + 4] = c.grid_step;
+            counters[p * 6 + 0] = c.steps;
+            counters[p * 6 + 1] = c.n_grid;
+            counters[p * 6 + 2] = c.n_rt;
+            counters[p * 6 + 3] = c.escaped;
+            counters[p * 6 + 4] = c.grid_step;
+            counters[p * 6 + 5] = c.opaque_step;
         }
         if (gate)
             for (int f = 0; f < nf; f++)

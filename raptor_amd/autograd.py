@@ -30,13 +30,17 @@ class RenderImage(torch.autograd.Function):
     @staticmethod
     def forward(ctx, params, tracer, inclinations, freqs):
         p = params.detach().cpu().double()
-        Rh = float(p[1])
         base = tracer.params
-        tracer.set_params(width=base.width, height=base.height, M_unit=float(torch.exp(p[0])),
-                          R_high=Rh, R_low=base.R_low, MBH=base.MBH, cam_size=base.cam_size_x,
-                          cam_size_y=base.cam_size_y, stepsize=base.stepsize,
-                          max_steps=base.max_steps)
-        img, dimg, _ = tracer.render(inclinations, freqs, grad=True)
+        keep = dict(width=base.width, height=base.height, R_low=base.R_low, MBH=base.MBH,
+                    cam_size=base.cam_size_x, cam_size_y=base.cam_size_y,
+                    stepsize=base.stepsize, max_steps=base.max_steps,
+                    integrator=base.integrator, emission=base.emission)
+        tracer.set_params(M_unit=float(torch.exp(p[0])), R_high=float(p[1]), **keep)
+        try:
+            img, dimg, _ = tracer.render(inclinations, freqs, grad=True)
+        finally:
+            # the tracer's own parameters again, whatever the render did
+            tracer.set_params(M_unit=base.M_unit, R_high=base.R_high, **keep)
         ctx.save_for_backward(torch.from_numpy(dimg))
         return torch.from_numpy(img)
 

@@ -26,6 +26,10 @@ HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 # Packed FP32 instructions are disabled for all device code.
 HIP_FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
+    # a * b + c is fused where one source expression writes it, and nowhere
+    # else: the same arithmetic in every kernel that inlines a function, so
+    # that launch shapes and frequency counts give the same bits
+    "-ffp-contract=on",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]

@@ -288,9 +288,15 @@ __device__ __forceinline__ void trace_rays(const Params &P, const TraceArgs &A)
     // resumed RK2 rays overlap each transfer sample with the next step's
     // first acceleration (ray_step_rk2_overlap), carried in A0
     constexpr bool kOverlap = RESUME && INT == kRK2;
+    // one frequency of the thermal fit: a ray whose FP32 optical depth met
+    // the ln 1000 gate where a different decision would show in the pixel
+    // (needs_exact) is traced again at once by its lane with the FP64
+    // transfer (xm); with several frequencies it is listed for exact_kernel
+    constexpr bool kInline = EMIS == kThSynch && NF == 1;
     Ray<NF, GRAD> R;
     double A0[4] = {0., 0., 0., 0.};
     int ray = 0, slot = 0;
+    bool xm = false;
     bool active = claim<RESUME>(P, A, ray, slot);
     bool exhausted = !active;
     if (active) {
@@ -302,19 +308,28 @@ __device__ __forceinline__ void trace_rays(const Params &P, const TraceArgs &A)
         if (active) {
             bool cont = ray_alive<NF, GRAD>(P, R);
             if (cont) {
-                if constexpr (kOverlap)
+                if constexpr (kInline && kOverlap)
+                    cont = ray_step_rk2_overlap_mixed<NF, GRAD, DIM3>(P, A.prims, R, A0, xm);
+                else if constexpr (kInline)
+                    cont = ray_step_mixed<NF, GRAD, DIM3, INT>(P, A.prims, R, xm);
+                else if constexpr (kOverlap)
                     cont = ray_step_rk2_overlap<NF, GRAD, DIM3, EMIS>(P, A.prims, R, A0);
                 else
                     cont = ray_step<NF, GRAD, DIM3, INT, EMIS>(P, A.prims, R);
             }
             if (!cont) {
-                write_ray<NF, GRAD, DIM3>(P, A, ray, R);
-                // its FP32 optical depth met the ln 1000 gate where a
-                // different decision would show in the pixel: listed for
-                // exact_kernel, which traces it again with the FP64 transfer
-                if (needs_exact<EMIS>(R))
-                    A.xlist[atomicAdd(A.xcount, 1u)] = slot;
-                active = false;
+                if (kInline && !xm && needs_exact<EMIS>(R)) {
+                    start_ray<RESUME>(P, A, ray, slot, R);
+                    if (kOverlap)
+                        geodesic_accel(R.pt, P.a, P.hslope, R.y + 4, A0);
+                    xm = true;
+                } else {
+                    write_ray<NF, GRAD, DIM3>(P, A, ray, R);
+                    if (!kInline && needs_exact<EMIS>(R))
+                        A.xlist[atomicAdd(A.xcount, 1u)] = slot;
+                    active = false;
+                    xm = false;
+                }
             }
         }
         // A wave holding a ray that has run kLongRay steps (a ray near the
@@ -359,12 +374,14 @@ __global__ __launch_bounds__(64, 3) void trace_kernel_3w(const Params P, const T
 }
 
 // The rays a trace kernel listed (needs_exact) traced again from their start
-// (the camera, or the saved ray at the grid's edge) with the FP64 transfer of
-// the thermal fit, and written over their FP32 outputs.  A kernel of its own:
-// its FP64 arithmetic needs more registers than the trace kernel can give
-// without losing its third wave (in that kernel it cost 6-60 spilled VGPRs).
-// About 1 ray in 3000 at 1024^2 and 230 GHz; consecutive entries go to
-// different waves, so that those few rays run on different SIMDs.
+// (the camera, or the saved ray at the grid's edge) by exact_retrace: the
+// frequencies whose FP32 optical depth met the ln 1000 gate where a different
+// decision would show in the pixel get the FP64 transfer's outputs, written
+// over their FP32 ones.  A kernel of its own: its FP64 arithmetic needs more
+// registers than the trace kernel can give without losing its third wave (in
+// that kernel it cost 6-60 spilled VGPRs).  About 1 ray in 3000 at 1024^2
+// and 230 GHz; consecutive entries go to different waves, so that those few
+// rays run on different SIMDs.
 template <int NF, bool GRAD, bool DIM3, int INT, bool RESUME>
 __global__ __launch_bounds__(64) void exact_kernel(const Params P, const TraceArgs A)
 {
@@ -379,22 +396,35 @@ __global__ __launch_bounds__(64) void exact_kernel(const Params P, const TraceAr
             continue;
         Ray<NF, GRAD> R;
         start_ray<RESUME>(P, A, ray, q, R);
-        while (ray_alive<NF, GRAD>(P, R) &&
-               ray_step<NF, GRAD, DIM3, INT, kThSynch, true>(P, A.prims, R)) {
+        Accum<NF, GRAD> x;
+        float wf[NF];
+        exact_retrace<NF, GRAD, DIM3, INT>(P, A.prims, R, x, wf);
+        const size_t o = (size_t)ray * P.nf;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            if (f < P.nf && wf[f] > kGateFlip) {
+                A.image[o + f] = (double)x.I[f];
+                if (GRAD) {
+                    A.dimage[(o + f) * 2 + 0] = x.dI[f][0];
+                    A.dimage[(o + f) * 2 + 1] = x.dI[f][1];
+                }
+            }
         }
-        write_ray<NF, GRAD, DIM3>(P, A, ray, R);
     }
 }
 
 // The energy spectrum (core.c:146-150): per camera and frequency the sum of
 // the image over its pixels, in two passes of fixed order -- partial sums of
 // kSpecBlocks contiguous pixel ranges, then their sum -- so that every call
-// gives the same bits.  In the first pass thread t of a block sums frequency
-// t % nf over every (256 / nf)-th pixel of the block's range (its reads of
-// image[pixel][f] are contiguous across the block), then one thread per
-// frequency adds those up in order.
+// gives the same bits.  Thread t of a block sums every 256th pixel of the
+// block's range from the t-th on, at each frequency (NF >= nf accumulators),
+// then a tree over the block's threads adds those up per frequency: the
+// order of each (camera, frequency) sum does not depend on how many
+// frequencies the image has, so batched and single-frequency renders give
+// the same spectrum.
 static constexpr int kSpecBlocks = 256;
 
+template <int NF>
 __global__ __launch_bounds__(256) void spectrum_partial(const double *img, int npix, int nf,
                                                         double *part)
 {
@@ -402,22 +432,32 @@ __global__ __launch_bounds__(256) void spectrum_partial(const double *img, int n
     const int cam = blockIdx.y, nb = gridDim.x;
     const int per = (npix + nb - 1) / nb;
     const int p0 = blockIdx.x * per, p1 = min(npix, p0 + per);
-    const int per_f = 256 / nf;   // threads per frequency
     const int t = threadIdx.x;
-    double sum = 0.;
-    if (t < per_f * nf) {
-        const double *im = img + (size_t)cam * npix * nf;
-        const int f = t % nf;
-        for (int i = p0 + t / nf; i < p1; i += per_f)
-            sum += im[(size_t)i * nf + f];
+    const double *im = img + (size_t)cam * npix * nf;
+    double sum[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+        sum[f] = 0.;
+    for (int i = p0 + t; i < p1; i += 256) {
+#pragma unroll
+        for (int f = 0; f < NF; f++)
+            if (f < nf)
+                sum[f] += im[(size_t)i * nf + f];
     }
-    red[t] = sum;
-    __syncthreads();
-    if (t < nf) {
-        double s = 0.;
-        for (int k = 0; k < per_f; k++)
-            s += red[t + k * nf];
-        part[((size_t)cam * nb + blockIdx.x) * nf + t] = s;
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        if (f < nf) {   // the same in every thread
+            red[t] = sum[f];
+            __syncthreads();
+            for (int w = 128; w > 0; w >>= 1) {
+                if (t < w)
+                    red[t] += red[t + w];
+                __syncthreads();
+            }
+            if (t == 0)
+                part[((size_t)cam * nb + blockIdx.x) * nf + f] = red[0];
+            __syncthreads();
+        }
     }
 }
 
@@ -781,10 +821,9 @@ static hipError_t launch_exact(rt_ctx *c, const Params &P, const TraceArgs &A)
 template <bool GRAD, bool DIM3, int INT, bool RESUME>
 static hipError_t dispatch_exact(rt_ctx *c, const Params &P, const TraceArgs &A)
 {
-    if (P.emission != kThSynch)
+    // one frequency: traced again by the trace kernel itself
+    if (P.emission != kThSynch || P.nf == 1)
         return hipSuccess;
-    if (P.nf == 1)
-        return launch_exact<1, GRAD, DIM3, INT, RESUME>(c, P, A);
     if (P.nf <= 4)
         return launch_exact<4, GRAD, DIM3, INT, RESUME>(c, P, A);
     if (P.nf <= 8)
@@ -1053,8 +1092,12 @@ static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *fr
     HIPCHK(c, e);
     if (d_spectrum) {
         const int nb = (int)std::min<long long>(kSpecBlocks, ((long long)P.npix + 1023) / 1024);
-        hipLaunchKernelGGL(spectrum_partial, dim3((unsigned)nb, (unsigned)ncam), dim3(256), 0,
-                           c->stream, (const double *)d_image, P.npix, nf, c->d_spec_part);
+        void (*part)(const double *, int, int, double *) =
+            nf == 1 ? spectrum_partial<1>
+                    : (nf <= 4 ? spectrum_partial<4>
+                               : (nf <= 8 ? spectrum_partial<8> : spectrum_partial<16>));
+        hipLaunchKernelGGL(part, dim3((unsigned)nb, (unsigned)ncam), dim3(256), 0, c->stream,
+                           (const double *)d_image, P.npix, nf, c->d_spec_part);
         hipLaunchKernelGGL(spectrum_final, dim3((unsigned)ncam), dim3(64), 0, c->stream,
                            (const double *)c->d_spec_part, nb, nf, d_spectrum);
         HIPCHK(c, hipGetLastError());

@@ -185,7 +185,83 @@ __device__ inline double drcp(double x)
 }
 __host__ inline double drcp(double x) { return 1. / x; }
 
-// theta = pi X2 + (1-h)/2 sin(2 pi X2) is evaluated as pi * (X2 + (1-h)/(2 pi)
+// FP64 elementary functions of the FP64 transfer (rays at the tau gate),
+// written from their series for the arguments that transfer gives them
+// (finite; exp's below 700 in magnitude, log's positive and normal):
+//  dexp    e^x = 2^k e^r, k = rint(x / ln 2), r = x - k ln 2 with ln 2 in two
+//          parts (each product exact inside its FMA), |r| <= ln 2 / 2; e^r by
+//          its Taylor series to r^13/13! (truncation < 5e-18) in Horner form,
+//          the last step fma(r, q, 1) rounded once, so that e^x - 1 for the
+//          Planck factor's tiny x keeps the rounding of a correctly rounded exp
+//  dexpm1  e^x - 1: the same series without its 1 for |x| <= 1/4
+//  dlog    ln x = e ln 2 + 2 atanh s, x = 2^e m with m in [sqrt(1/2), sqrt 2),
+//          s = (m - 1)/(m + 1), |s| <= 0.1716: atanh's series to s^25/25
+//          (truncation < 2e-22)
+//  dsqrt   x / sqrt(x) from the hardware reciprocal square root refined by
+//          two Newton steps, then one correction of the product
+// Each is a few dozen instructions, against a few hundred for the general
+// library functions with their special cases.
+constexpr double kLn2Hi = 0.6931471805599453, kLn2Lo = 2.3190468138462996e-17;
+
+__host__ __device__ inline double exp_series_q(double r)   // (e^r - 1) / r
+{
+    double q = 1. / 6227020800.;
+    q = fma(q, r, 1. / 479001600.);
+    q = fma(q, r, 1. / 39916800.);
+    q = fma(q, r, 1. / 3628800.);
+    q = fma(q, r, 1. / 362880.);
+    q = fma(q, r, 1. / 40320.);
+    q = fma(q, r, 1. / 5040.);
+    q = fma(q, r, 1. / 720.);
+    q = fma(q, r, 1. / 120.);
+    q = fma(q, r, 1. / 24.);
+    q = fma(q, r, 1. / 6.);
+    q = fma(q, r, 0.5);
+    return fma(q, r, 1.);
+}
+
+__host__ __device__ inline double dexp(double x)
+{
+    const double k = rint(x * 1.4426950408889634);
+    const double r = fma(-k, kLn2Lo, fma(-k, kLn2Hi, x));
+    return ldexp(fma(r, exp_series_q(r), 1.), (int)k);
+}
+
+__host__ __device__ inline double dexpm1(double x)
+{
+    return fabs(x) <= 0.25 ? x * exp_series_q(x) : dexp(x) - 1.;
+}
+
+__host__ __device__ inline double dlog(double x)
+{
+    int e;
+    double m = frexp(x, &e);
+    if (m < 0.7071067811865476) {
+        m *= 2.;
+        e -= 1;
+    }
+    const double t = (m - 1.) * drcp(m + 1.);
+    const double t2 = t * t;
+    double q = 1. / 25.;
+    for (int n = 23; n >= 3; n -= 2)
+        q = fma(q, t2, 1. / n);
+    q = fma(q, t2, 1.);
+    const double fe = (double)e;
+    return fma(fe, kLn2Hi, fma(fe, kLn2Lo, 2. * t * q));
+}
+
+__device__ inline double dsqrt(double x)
+{
+    double r = __builtin_amdgcn_rsq(x);
+    r = r * fma(-0.5 * x * r, r, 1.5);
+    r = r * fma(-0.5 * x * r, r, 1.5);
+    double y = x * r;
+    y = fma(0.5 * r, fma(-y, y, x), y);
+    return x > 0. ? y : (x == 0. ? 0. : NAN);
+}
+__host__ inline double dsqrt(double x) { return sqrt(x); }
+
+// theta = pi X2 + (1-h)/2 sin(2 pi X2) is evaluated as pi * (X2 + (1-h)/(2 pi)// theta = pi X2 + (1-h)/2 sin(2 pi X2) is evaluated as pi * (X2 + (1-h)/(2 pi)
 // sin(2 pi X2)), so both sine/cosine pairs take arguments in units of pi and
 // reduce exactly (sincospi) instead of through a general argument reduction.
 __host__ __device__ inline Point make_point(double X1, double X2, double h)
@@ -1012,9 +1088,11 @@ __host__ __device__ inline void kahan_add(float &s, float &c, float x)
 // dl = |dlambda| R_g:
 //   dtau = nu_p alpha_nu dl / nu = g alpha_nu dl
 //   dI nu^3 = (j_nu / nu_p^2) exp(-tau) (dl / nu) nu^3 = j_nu exp(-tau) dl / g^2
+// wf (the second trace of exact_retrace): each frequency's share of the
+// samples in the gate window, instead of their sum in acc.wrel.
 template <int NF, bool GRAD, int EMIS = kThSynch>
 __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F, double dlabs,
-                                                Accum<NF, GRAD> &acc)
+                                                Accum<NF, GRAD> &acc, float *wf = nullptr)
 {
     const float lnth = 6.907755278982137f;   // log(1000.)
     const float dl = (float)(dlabs * P.Rg);
@@ -1089,7 +1167,11 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
             // I with this sample added either way; 0 (a first sample at the
             // gate) gives an infinite share, which flags the ray
             const float Iw = tau < lnth ? acc.I[f] : acc.I[f] + dI;
-            acc.wrel += dI * frcp(Iw);
+            const float share = dI * frcp(Iw);
+            if (wf)
+                wf[f] += share;
+            else
+                acc.wrel += share;
         }
     }
 }
@@ -1112,44 +1194,47 @@ __host__ __device__ inline bool fluid_sample_f64(const Params &P, const float4 *
     const double rho = v[0], uu = v[1];
     const MetricDD g = metric_dd(pt, P.a, P.hslope);
     const double r = pt.r;
-    const double rho2 = r * r + P.a * P.a * (pt.c * pt.c);
-    const double gcon00 = -1. - 2. * r / rho2, gcon01 = 2. / rho2;   // metric.c:207-231
+    const double irho2 = drcp(r * r + P.a * P.a * (pt.c * pt.c));
+    const double gcon00 = -1. - 2. * r * irho2, gcon01 = 2. * irho2;   // metric.c:207-231
     const double V1 = v[2], V2 = v[3], V3 = v[4];
     const double VdotV = g.g11 * V1 * V1 + 2. * g.g13 * V1 * V3 + g.g22 * V2 * V2 + g.g33 * V3 * V3;
-    const double Vfac = sqrt(-1. / gcon00 * (1. + fabs(VdotV)));
+    const double Vfac = dsqrt(-drcp(gcon00) * (1. + fabs(VdotV)));
     const double u[4] = {-Vfac * gcon00, V1 - Vfac * gcon01, V2, V3};
     double ud[4];
     lower(g, u, ud);
     const double UdotBp = ud[1] * v[5] + ud[2] * v[6] + ud[3] * v[7];
-    const double b[4] = {UdotBp, (v[5] + u[1] * UdotBp) / u[0], (v[6] + u[2] * UdotBp) / u[0],
-                         (v[7] + u[3] * UdotBp) / u[0]};
+    const double iu0 = drcp(u[0]);
+    const double b[4] = {UdotBp, (v[5] + u[1] * UdotBp) * iu0, (v[6] + u[2] * UdotBp) * iu0,
+                         (v[7] + u[3] * UdotBp) * iu0};
     double bd[4];
     lower(g, b, bd);
     const double Bsq = b[0] * bd[0] + b[1] * bd[1] + b[2] * bd[2] + b[3] * bd[3];
     F.ne = rho * P.Ne_unit;
-    F.B = sqrt(Bsq) * P.B_unit + 1e-20;
-    const double beta = uu * (4. / 3. - 1.) / (0.5 * (Bsq + 1e-20));
+    F.B = dsqrt(Bsq) * P.B_unit + 1e-20;
+    const double beta = uu * (4. / 3. - 1.) * drcp(0.5 * (Bsq + 1e-20));
     const double b2 = beta * beta;
-    const double trat = P.R_high * b2 / (1. + b2) + P.R_low / (1. + b2);
-    const double gam2 = 0.5 * ((1. + 2. / 3. * (trat + 1.) / (trat + 2.)) + 4. / 3.);
+    const double w = drcp(1. + b2);
+    const double trat = P.R_high * b2 * w + P.R_low * w;
+    const double itr2 = drcp(trat + 2.), itr1 = drcp(1. + trat);
+    const double gam2 = 0.5 * ((1. + 2. / 3. * (trat + 1.) * itr2) + 4. / 3.);
     const double mr = kProtonMass / kElectronMass;
-    F.thetae = (uu / rho) * ((gam2 - 1.) * mr / (1. + trat));
-    const double dgam2 = (1. / 3.) / ((trat + 2.) * (trat + 2.));
-    F.dthetae = (uu / rho) * (mr * (dgam2 * (1. + trat) - (gam2 - 1.)) /
-                              ((1. + trat) * (1. + trat))) * (b2 / (1. + b2));
+    const double uor = uu * drcp(rho);
+    F.thetae = uor * ((gam2 - 1.) * mr * itr1);
+    const double dgam2 = (1. / 3.) * (itr2 * itr2);
+    F.dthetae = uor * (mr * (dgam2 * (1. + trat) - (gam2 - 1.)) * (itr1 * itr1)) * (b2 * w);
     if (F.thetae < 0.) {
         F.thetae = 1e-20;
         F.dthetae = 0.;
     }
-    if (Bsq / rho > 1.)
+    if (Bsq * drcp(rho) > 1.)
         return false;
     double kd[4];
     lower(g, y + 4, kd);
     const double ku = kd[0] * u[0] + kd[1] * u[1] + kd[2] * u[2] + kd[3] * u[3];
     double mu = 0.;
     if (F.B != 0.) {
-        mu = (kd[0] * b[0] + kd[1] * b[1] + kd[2] * b[2] + kd[3] * b[3]) /
-             (fabs(ku) * (F.B / P.B_unit));
+        mu = (kd[0] * b[0] + kd[1] * b[1] + kd[2] * b[2] + kd[3] * b[3]) *
+             drcp(fabs(ku) * (F.B / P.B_unit));
         if (fabs(mu) > 1.)
             mu /= fabs(mu);
     }
@@ -1177,11 +1262,11 @@ __host__ __device__ inline bool transfer_sample_f64(const Params &P,
         return false;
     const double lnth = 6.907755278982137;   // log(1000.)
     const double dl = dlabs * P.Rg;
-    const double sth = sqrt(1. - F.mu * F.mu);
+    const double sth = dsqrt(1. - F.mu * F.mu);
     const double nu_c = kElectronCharge * F.B / (2. * kPi * kElectronMass * kC);
     const double nu_s = (2. / 9.) * nu_c * F.thetae * F.thetae * sth;
     const bool on = !(fabs(sth) < 1.e-10) && F.thetae > 1e-4;
-    const double xk = kPlanck / (kElectronMass * kC * kC * F.thetae);   // x = h nu_p / k T_e
+    const double xk = kPlanck / (kElectronMass * kC * kC) * drcp(F.thetae);   // x = h nu_p / k T_e
     const double c1112 = 1.8877486253633868;                            // 2^(11/12)
 #pragma unroll
     for (int f = 0; f < NF; f++) {
@@ -1196,28 +1281,30 @@ __host__ __device__ inline bool transfer_sample_f64(const Params &P,
         const double x = xk * nu_p;
         double j = 0., s6 = 0.;
         if (on && nu_p != 1.) {
-            const double X = nu_p / nu_s;
-            s6 = exp(log(X) * (1. / 6.));          // X^(1/6)
+            const double X = nu_p * drcp(nu_s);
+            s6 = dexp(dlog(X) * (1. / 6.));        // X^(1/6)
             const double ff = s6 * s6 * s6 + c1112 * s6;
-            j = F.ne * (1.4142135623730951 * kPi / 3.) * nu_s / (2. * F.thetae * F.thetae) *
-                (ff * ff) * exp(-(s6 * s6)) * kE2C;
+            j = F.ne * (1.4142135623730951 * kPi / 3.) * nu_s * drcp(2. * F.thetae * F.thetae) *
+                (ff * ff) * dexp(-(s6 * s6)) * kE2C;
         }
-        // alpha = j (e^2/ch) / B_nu / (e^2/c), B_nu = 2 nu^3 / c^2 / (e^x - 1): 0 for
-        // j = 0 unless B_nu is 0 (e^x overflows), where the reference's 0/0 is NaN
-        const double Bnu = 2. * nu_p * nu_p * nu_p / (kC * kC) * (1. / (exp(x) - 1.));
-        const double alpha = j != 0. ? j * kE2CH / Bnu / kE2C : (Bnu == 0. ? NAN : 0.);
-        const double dtau = nu_p / nu * alpha * dl;
+        // alpha = j (e^2/ch) / B_nu / (e^2/c) with B_nu = 2 nu^3 / c^2 / (e^x - 1),
+        // written as a product: where e^x - 1 is 0 (B_nu infinite) it is 0, and
+        // where e^x overflows (B_nu 0) it is infinite, or NaN for j = 0 -- the
+        // reference's divisions give the same
+        const double alpha = j * (kE2CH / kE2C) * (dexp(x) - 1.) * (0.5 * kC * kC) *
+                             drcp(nu_p * nu_p * nu_p);
+        const double dtau = nu_p * drcp(nu) * alpha * dl;
         double tau = tau_f64(acc.tau[f], acc.ctau[f]) + dtau;
         set_tau_f64(acc.tau[f], acc.ctau[f], tau);
-        const double gi = nu / nu_p;
-        const double dI = j * exp(-tau) * dl * (gi * gi);
+        const double gi = nu * drcp(nu_p);
+        const double dI = j * dexp(-tau) * dl * (gi * gi);
         double dlnj0 = 0., dlnj1 = 0., dlna1 = 0.;
         if (GRAD && j != 0.) {
             const double cb = s6 * s6, sx = cb * s6;
-            const double Dx = 2. * (0.5 * sx + c1112 / 6. * s6) / (sx + c1112 * s6) - cb / 3.;
-            const double dlnB = 0.5 * (F.B - 1e-20) / F.B;
-            const double dlnT = F.dthetae / F.thetae;
-            const double dlnBnu = x / (-expm1(-x));
+            const double Dx = 2. * (0.5 * sx + c1112 / 6. * s6) * drcp(sx + c1112 * s6) - cb / 3.;
+            const double dlnB = 0.5 * (F.B - 1e-20) * drcp(F.B);
+            const double dlnT = F.dthetae * drcp(F.thetae);
+            const double dlnBnu = x * drcp(-dexpm1(-x));
             dlnj0 = 1. + dlnB * (1. - Dx);
             dlnj1 = -2. * Dx * dlnT;
             dlna1 = dlnj1 - dlnBnu * dlnT;
@@ -1237,6 +1324,36 @@ __host__ __device__ inline bool transfer_sample_f64(const Params &P,
     return true;
 }
 
+// A ray whose optical depth has passed the gate at every frequency adds
+// nothing more to its pixel: tau only grows (dtau >= 0, or NaN, after which
+// no sample adds either) and a sample adds to I and its tangents only while
+// tau < ln 1000 (radiative_transfer.c:90-95).  Such a ray ends there
+// (Ray::terminate), with the outputs of a trace to the horizon or the outer
+// edge.  The FP32 transfer ends 2 kGateMargin past the gate, beyond the gate
+// window (its Kahan-summed tau can step back by an ulp or two); the FP64 one
+// at the gate.
+constexpr float kOpaqueTau = 6.907755278982137f + 2.f * kGateMargin;
+
+template <int NF, bool GRAD>
+__host__ __device__ inline bool opaque(const Params &P, const Accum<NF, GRAD> &acc)
+{
+    bool o = true;
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+        o = o && (f >= P.nf || !(acc.tau[f] < kOpaqueTau));
+    return o;
+}
+
+template <int NF, bool GRAD>
+__host__ __device__ inline bool opaque_f64(const Params &P, const Accum<NF, GRAD> &acc)
+{
+    bool o = true;
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+        o = o && (f >= P.nf || !(tau_f64(acc.tau[f], acc.ctau[f]) < 6.907755278982137));
+    return o;
+}
+
 // A ray being traced.
 template <int NF, bool GRAD>
 struct Ray {
@@ -1251,8 +1368,8 @@ struct Ray {
 
 // Whether a ray traced with the FP32 transfer is to be traced again with the
 // FP64 one: its FP32 tau passed the gate within kGateMargin at samples
-// carrying more than kGateFlip of its intensity (the thermal fit, whose FP64
-// form is above).
+// carrying more than kGateFlip of its intensity, summed over its frequencies
+// (the thermal fit, whose FP64 form is above).
 template <int EMIS, int NF, bool GRAD>
 __host__ __device__ inline bool needs_exact(const Ray<NF, GRAD> &R)
 {
@@ -1329,13 +1446,16 @@ __host__ __device__ inline void ray_sample(const Params &P, const float4 *__rest
     if (in_grid(P, R.y)) {
         if constexpr (EXACT) {
             static_assert(EMIS == kThSynch, "the FP64 transfer is the thermal fit's");
-            if (transfer_sample_f64<NF, GRAD, DIM3>(P, prims, R.pt, R.y, fabs(dl), R.acc))
+            if (transfer_sample_f64<NF, GRAD, DIM3>(P, prims, R.pt, R.y, fabs(dl), R.acc)) {
                 R.samples++;
+                R.terminate = R.terminate || opaque_f64(P, R.acc);
+            }
         } else {
             Fluid F;
             if (fluid_sample<DIM3>(P, prims, R.pt, R.y, F)) {
                 R.samples++;
                 transfer_sample<NF, GRAD, EMIS>(P, F, fabs(dl), R.acc);
+                R.terminate = R.terminate || opaque(P, R.acc);
             }
         }
     }
@@ -1353,6 +1473,40 @@ __host__ __device__ inline bool ray_step(const Params &P, const float4 *__restri
         return false;
     ray_sample<NF, GRAD, DIM3, EMIS, EXACT>(P, prims, R, dl);
     return true;
+}
+
+// The second trace of a ray that needs_exact, from its start state in R:
+// the FP32 transfer again, now with each frequency's share of the samples in
+// the gate window in wf, beside the FP64 transfer into x on the same
+// geodesic.  A frequency whose own share exceeds kGateFlip -- the decision a
+// launch of that frequency alone makes -- takes x's outputs (FP64 tau and
+// tangents, intensity summed in FP32), the others keep the FP32 ones (R.acc,
+// the same as the first trace's), so that batched and single-frequency
+// launches give the same bits.
+template <int NF, bool GRAD, bool DIM3, int INT>
+__host__ __device__ inline void exact_retrace(const Params &P, const float4 *__restrict__ prims,
+                                              Ray<NF, GRAD> &R, Accum<NF, GRAD> &x,
+                                              float (&wf)[NF])
+{
+    x.clear();
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+        wf[f] = 0.f;
+    while (ray_alive<NF, GRAD>(P, R)) {
+        double dl;
+        if (!ray_advance<INT>(P, R, dl))
+            break;
+        if (in_grid(P, R.y)) {
+            Fluid F;
+            if (fluid_sample<DIM3>(P, prims, R.pt, R.y, F)) {
+                R.samples++;
+                transfer_sample<NF, GRAD, kThSynch>(P, F, fabs(dl), R.acc, wf);
+            }
+            transfer_sample_f64<NF, GRAD, DIM3>(P, prims, R.pt, R.y, fabs(dl), x);
+            R.terminate = R.terminate || (opaque(P, R.acc) && opaque_f64(P, x));
+        }
+        R.steps++;
+    }
 }
 
 // The camera-to-grid leg of a ray (integrate_geodesic, integrator.c:228-277,
@@ -1432,6 +1586,64 @@ __host__ __device__ inline bool ray_step_rk2_overlap(const Params &P,
     if (inside && sampled) {
         R.samples++;
         transfer_sample<NF, GRAD, EMIS>(P, F, fabs(dl), R.acc);
+        R.terminate = R.terminate || opaque(P, R.acc);
+    }
+    R.steps++;
+    return true;
+}
+
+// ray_step and ray_step_rk2_overlap with the transfer chosen per lane: the
+// FP64 one (transfer_sample_f64) where `exact`, the FP32 one elsewhere; the
+// geodesic step is shared, so a wave with a lane in its second trace
+// (needs_exact) runs the FP64 sample beside the others' FP32 sample.
+template <int NF, bool GRAD, bool DIM3, int INT>
+__host__ __device__ inline bool ray_step_mixed(const Params &P, const float4 *__restrict__ prims,
+                                               Ray<NF, GRAD> &R, bool exact)
+{
+    double dl;
+    if (!ray_advance<INT>(P, R, dl))
+        return false;
+    if (in_grid(P, R.y)) {
+        if (exact) {
+            if (transfer_sample_f64<NF, GRAD, DIM3>(P, prims, R.pt, R.y, fabs(dl), R.acc)) {
+                R.samples++;
+                R.terminate = R.terminate || opaque_f64(P, R.acc);
+            }
+        } else {
+            Fluid F;
+            if (fluid_sample<DIM3>(P, prims, R.pt, R.y, F)) {
+                R.samples++;
+                transfer_sample<NF, GRAD, kThSynch>(P, F, fabs(dl), R.acc);
+                R.terminate = R.terminate || opaque(P, R.acc);
+            }
+        }
+    }
+    R.steps++;
+    return true;
+}
+
+template <int NF, bool GRAD, bool DIM3>
+__host__ __device__ inline bool ray_step_rk2_overlap_mixed(const Params &P,
+                                                           const float4 *__restrict__ prims,
+                                                           Ray<NF, GRAD> &R, double A0[4],
+                                                           bool exact)
+{
+    double dl;
+    if (!ray_advance_from(P, R, dl, A0))
+        return false;
+    const bool inside = in_grid(P, R.y);
+    Fluid F;
+    const bool sampled = fluid_sample<DIM3>(P, prims, R.pt, R.y, F);
+    geodesic_accel(R.pt, P.a, P.hslope, R.y + 4, A0);
+    if (exact) {
+        if (inside && transfer_sample_f64<NF, GRAD, DIM3>(P, prims, R.pt, R.y, fabs(dl), R.acc)) {
+            R.samples++;
+            R.terminate = R.terminate || opaque_f64(P, R.acc);
+        }
+    } else if (inside && sampled) {
+        R.samples++;
+        transfer_sample<NF, GRAD, kThSynch>(P, F, fabs(dl), R.acc);
+        R.terminate = R.terminate || opaque(P, R.acc);
     }
     R.steps++;
     return true;

@@ -84,8 +84,18 @@ int get_fluid_params(rt_ctx *ctx, const rt_grid *g, const double X[4], double *N
                      double Ucon[4], int *IN_VOLUME, double gcov[4][4], double gcon[4][4])
 {
     double out[14];
-    if (rt_fluid_params(ctx, X, 1, out) != RT_OK)
-        return -1;
+    if (rt_fluid_params(ctx, X, 1, out) != RT_OK) {
+        /* no emission, as for the sigma cut (raptor_harm_model.c:285-290),
+         * so that callers testing the result with if() skip the sample; the
+         * error shows as IN_VOLUME = 0 (the reference always sets 1) and in
+         * rt_last_error(ctx) */
+        *Ne = *Thetae = *B = 1e-20;
+        *beta = *Bern = 0.;
+        for (int i = 0; i < 4; i++)
+            Ucon[i] = Bcon[i] = 0.;
+        *IN_VOLUME = 0;
+        return 0;
+    }
     *Ne = out[0];
     *Thetae = out[1];
     *B = out[2];
@@ -107,21 +117,18 @@ int calculate_image(rt_ctx *ctx, const rio_model *m, double incl, double **inten
     size_t npix = (size_t)m->IMG_WIDTH * m->IMG_HEIGHT;
     int nf = m->n_freq;
     double *img = malloc(npix * nf * sizeof(double));
-    double *spec = malloc(nf * sizeof(double));
-    if (!img || !spec) {
-        free(img);
-        free(spec);
+    if (!img)
         return RT_E_NOMEM;
-    }
-    int rc = rt_render(ctx, &incl, 1, frequencies, nf, img, spec);
+    int rc = rt_render(ctx, &incl, 1, frequencies, nf, img, NULL);
     if (rc == RT_OK) {
+        /* core.c:136-150: the image, and the spectrum summed pixel by pixel
+         * in the reference's order, so that the spectrum file has its digits */
         for (size_t i = 0; i < npix; i++)
-            for (int f = 0; f < nf; f++)
+            for (int f = 0; f < nf; f++) {
                 intensityfield[i][f] = img[i * nf + f];
-        for (int f = 0; f < nf; f++)
-            energy_spectrum[f] += spec[f];
+                energy_spectrum[f] += img[i * nf + f];
+            }
     }
     free(img);
-    free(spec);
     return rc;
 }

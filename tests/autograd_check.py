@@ -47,17 +47,26 @@ def main():
     wn = np.random.default_rng(3).uniform(0.5, 1.5, W * W) * smooth
     w = torch.from_numpy(wn.reshape(1, W * W, 1))
     with rt.Tracer(0) as tr:
-        tr.set_params(width=W, height=W, M_unit=M0, R_high=R0)
+        # the tracer's own parameters differ from the ones the render is asked
+        # for (and use another integrator): the render takes its own and
+        # keeps the integrator, and the tracer's come back afterwards
+        tr.set_params(width=W, height=W, M_unit=3 * M0, R_high=2 * R0, integrator=rt.INT_VERLET)
         tr.load_dump(dump)
         params = torch.tensor([math.log(M0), R0], dtype=torch.float64, requires_grad=True)
+        img_v = autograd.render_image(params, tr, [incl], [freq]).detach()
+        restored = (tr.params.M_unit == 3 * M0 and tr.params.R_high == 2 * R0 and
+                    tr.params.integrator == rt.INT_VERLET)
+        tr.set_params(width=W, height=W, M_unit=M0, R_high=R0)
         img = autograd.render_image(params, tr, [incl], [freq])
+        verlet_differs = bool((img_v != img.detach()).any())
         loss = (img * w).sum()
         loss.backward()
         grad = params.grad.tolist()
 
     fd = [float((fdpix[q] * wn).sum()) for q in range(2)]
     print(json.dumps({"loss": float(loss), "loss_oracle": float((oracle_img(M0, R0) * wn).sum()),
-                      "grad": grad, "fd": fd, "smooth_frac": float(smooth.mean())}))
+                      "grad": grad, "fd": fd, "smooth_frac": float(smooth.mean()),
+                      "params_restored": restored, "integrator_kept": verlet_differs}))
 
 
 if __name__ == "__main__":

@@ -55,6 +55,22 @@ def rel_err(a, b, floor_frac=1e-6):
     return np.abs(a - b) / np.maximum(np.abs(b), floor)
 
 
+def counters_agree(cnt, ocnt):
+    """Per ray, whether the kernels' counters (steps, samples) agree with the
+    oracle's (columns steps, samples in the grid, samples past the sigma cut,
+    escaped, grid step, opaque step).  A ray whose optical depth passes
+    ln 1000 at every frequency ends there in the kernels, since no later
+    sample reaches its pixel: its steps lie after the oracle's opaque step
+    and at most at its full trace's (the FP32 transfer ends 2e-3 past the
+    gate, the FP64 one at it).  Other rays take the oracle's steps and
+    samples."""
+    steps, samples = cnt[:, 0], cnt[:, 1]
+    op = ocnt[:, 5]
+    full = (steps == ocnt[:, 0]) & (samples == ocnt[:, 2])
+    ended = (op >= 0) & (steps > op) & (steps <= ocnt[:, 0]) & (samples <= ocnt[:, 2])
+    return full | ended
+
+
 # Spacetimes and step sizes beyond the headline workload: Schwarzschild, a
 # near-extremal spin, the unsqueezed grid (h = 1), a negative spin (on the
 # a = 0.5 torus's primitives: parity between two codes needs no consistent

@@ -41,8 +41,32 @@ static void trace_to_end(const Params &P, const float4 *prims, Ray<NF, GRAD> &R,
     }
     if constexpr (!EXACT && EMIS == kThSynch) {
         if (retrace && needs_exact<EMIS>(R)) {
+            // exact_kernel: the second trace, FP64 outputs where a frequency's
+            // own window share asks for them; the counters stay the first
+            // trace's with several frequencies (exact_kernel writes the image
+            // only) and are the second's with one (the trace kernel's lane
+            // traces it again itself)
+            const int steps1 = R.steps, samples1 = R.samples;
             restart(R);
-            trace_to_end<NF, GRAD, INT, EMIS, true>(P, prims, R, restart);
+            Accum<NF, GRAD> x;
+            float wf[NF];
+            if (P.n3 > 1)
+                exact_retrace<NF, GRAD, true, INT>(P, prims, R, x, wf);
+            else
+                exact_retrace<NF, GRAD, false, INT>(P, prims, R, x, wf);
+            if (P.nf > 1) {
+                R.steps = steps1;
+                R.samples = samples1;
+            }
+            for (int f = 0; f < P.nf; f++) {
+                if (wf[f] > kGateFlip) {
+                    R.acc.I[f] = x.I[f];
+                    if (GRAD) {
+                        R.acc.dI[f][0] = x.dI[f][0];
+                        R.acc.dI[f][1] = x.dI[f][1];
+                    }
+                }
+            }
         }
     }
 }

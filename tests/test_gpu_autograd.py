@@ -19,6 +19,7 @@ def test_autograd_gradient_matches_oracle_differences():
     assert r.returncode == 0, r.stderr[-2000:]
     out = json.loads(r.stdout.strip().splitlines()[-1])
     assert out["smooth_frac"] > 0.85
+    assert out["params_restored"] and out["integrator_kept"]
     assert out["loss"] == pytest.approx(out["loss_oracle"], rel=1e-4)
     for g, f in zip(out["grad"], out["fd"]):
         assert g == pytest.approx(f, rel=2e-3)

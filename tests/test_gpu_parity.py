@@ -7,7 +7,8 @@ near-FP64 rounding.
 import numpy as np
 import pytest
 
-from conftest import FREQ, INCL, M_UNIT, R_HIGH, SPACETIME_CASES, rel_err, spacetime_case
+from conftest import (FREQ, INCL, M_UNIT, R_HIGH, SPACETIME_CASES, counters_agree, rel_err,
+                      spacetime_case)
 from oracle import oracle as O
 from raptor_amd import rt
 
@@ -123,8 +124,7 @@ def test_ray_counters_match_oracle(tracer, dump128):
         tracer.free(d_img)
         tracer.free(d_cnt)
     _, oc = O.render(P, dump128.prims, [FREQ], counters=True)
-    assert (cnt[:, 0] == oc[:, 0]).mean() > 0.99
-    assert (cnt[:, 1] == oc[:, 2]).mean() > 0.99
+    assert counters_agree(cnt, oc).mean() > 0.99
 
 
 def test_gradients_match_oracle(tracer, dump128):
@@ -143,8 +143,8 @@ def test_gradients_match_oracle(tracer, dump128):
 @pytest.mark.parametrize("nf", [3, 8])
 def test_gradients_several_frequencies_match_oracle(tracer, dump128, nf):
     """The tangent pass of the 4- and 8-frequency kernels against the
-    oracle's tangents, every frequency.  Pixels near the ln 1000 gate are held
-    apart per pixel, as for the images.  At 86 GHz the torus is thick enough
+    oracle's tangents, every frequency, every pixel (rays at the ln 1000 gate
+    take the FP64 transfer's tangents).  At 86 GHz the torus is thick enough
     that some pixels hardly depend on M_unit: their tangent, a sum of terms
     of both signs, is ~1e-5 of the image's largest, and its relative error
     says little; every pixel is held to 1e-3 in the logarithmic derivative
@@ -152,16 +152,16 @@ def test_gradients_several_frequencies_match_oracle(tracer, dump128, nf):
     P = _setup(tracer, dump128, 32)
     freqs = list(86e9 * (690.0 / 86.0) ** (np.arange(nf) / (nf - 1)))
     img, dimg, _ = tracer.render([INCL], freqs, grad=True)
-    oimg, odimg, gate = O.render(P, dump128.prims, freqs, grad=True, gate=True)
+    oimg, odimg = O.render(P, dump128.prims, freqs, grad=True)
     for f in range(nf):
         assert rel_err(img[0, :, f], oimg[:, f]).max() < PIX_TOL, f
         ipix = np.maximum(oimg[:, f], 1e-6 * oimg[:, f].max())
         for q, p in enumerate((1.0, R_HIGH)):   # d/d ln M_unit, R_high d/d R_high
             g, o = dimg[0, :, f, q], odimg[:, f, q]
             assert abs(g.sum() / o.sum() - 1) < 1e-4, (f, q)
-            r = rel_err(g, o, 1e-5)[off]
+            r = rel_err(g, o, 1e-5)
             assert np.mean(r < 1e-3) > 0.99, (f, q)
-            assert (np.abs(g - o) * p / ipix)[off].max() < 1e-3, (f, q)
+            assert (np.abs(g - o) * p / ipix).max() < 1e-3, (f, q)
 
 
 def test_multi_frequency_and_camera_batch(tracer, dump128):
@@ -293,7 +293,7 @@ def test_other_spacetimes_and_step_sizes(tracer, case):
     r = _report(case, gpu, ora[:, 0])
     assert abs(gpu.sum() / ora[:, 0].sum() - 1) < FLUX_TOL
     assert r.max() < PIX_TOL
-    assert (cnt[:, 0] == ocnt[:, 0]).mean() > 0.99 and (cnt[:, 1] == ocnt[:, 2]).mean() > 0.99
+    assert counters_agree(cnt, ocnt).mean() > 0.99
 
 
 @pytest.mark.parametrize("integrator", [rt.INT_RK4, rt.INT_VERLET])
@@ -319,7 +319,7 @@ def test_other_integrators(tracer, dump128, integrator):
     r = _report("integrator %d" % integrator, gpu, ora[:, 0])
     assert abs(gpu.sum() / ora[:, 0].sum() - 1) < FLUX_TOL
     assert r.max() < PIX_TOL
-    assert (cnt[:, 0] == ocnt[:, 0]).mean() > 0.99 and (cnt[:, 1] == ocnt[:, 2]).mean() > 0.99
+    assert counters_agree(cnt, ocnt).mean() > 0.99
 
 
 def test_bad_integrator_rejected(tracer):
