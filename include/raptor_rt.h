@@ -139,6 +139,27 @@ int rt_render_device(rt_ctx *ctx, const double *inclinations, int n_cam,
                      const double *freqs, int n_freq, double *d_image, double *d_dimage,
                      rt_ray_counters *d_counters, double *d_spectrum);
 
+/* rt_render_device without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and outputs, bit for bit.  The work queue,
+ * the FP64 re-trace's ray list and the spectrum's partial sums live in one
+ * static device array of the library (18 MiB per GPU, held until the
+ * process ends) instead of buffers the library allocates; hence at most
+ * RT_NOALLOC_MAX_RAYS work items per call (n_cam x W x H, in whole claim
+ * tiles), RT_E_INVALID beyond.  It traces every ray from the camera in one
+ * launch (the launch configuration's approach is not used) and claims rays
+ * in the centre-out order only if an earlier rt_render_device on this
+ * context built that order for the image shape, in tiled scan order
+ * otherwise.  Every call on one GPU uses that GPU's array: calls that
+ * overlap in time on one GPU (from several contexts or threads) race and
+ * give wrong results. */
+#define RT_NOALLOC_MAX_RAYS (1 << 22)
+int rt_render_device_noalloc(rt_ctx *ctx, const double *inclinations, int n_cam,
+                             const double *freqs, int n_freq, double *d_image,
+                             double *d_dimage, rt_ray_counters *d_counters,
+                             double *d_spectrum);
+
 /* Wait for the context's stream. */
 int rt_synchronize(rt_ctx *ctx);
 

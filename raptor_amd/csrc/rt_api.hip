@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -664,6 +665,9 @@ struct rt_ctx {
     // spectrum: partial sums [camera][kSpecBlocks][frequency] and, for host
     // renders, the spectrum itself; made by rt_create
     double *d_spec_part = nullptr, *d_spec = nullptr;
+    // this device's `scratch` (rt_render_device_noalloc), from
+    // hipGetSymbolAddress at rt_create
+    void *d_scratch = nullptr;
     double *d_image = nullptr, *d_dimage = nullptr;
     size_t image_cap = 0, dimage_cap = 0;
     // the kappa absorptivity's hypergeometric table (kappa_table), made once
@@ -698,6 +702,44 @@ static constexpr size_t kMaxOrderTables = 8;
 // Work items per approach + trace launch pair with the split launch: 2^23
 // saved rays are 1 GB; larger batches run in chunks of this size.
 static constexpr int kMaxChunk = 1 << 23;
+
+// rt_render_device_noalloc's device memory: one static array per GPU, held
+// until the process ends, in place of the buffers rt_render_device
+// allocates -- the work queue's two counters, the FP64 re-trace's list (one
+// int per work item) and the spectrum's partial sums.  Every call on a GPU
+// shares its array: calls that overlap on one GPU race.  Host code uses the
+// address hipGetSymbolAddress returns (scratch_address), never the name.
+static constexpr int kNoAllocRays = RT_NOALLOC_MAX_RAYS;
+static constexpr size_t kScratchQueue = 0;     // 2 unsigned ints, 256 bytes reserved
+static constexpr size_t kScratchXlist = 256;
+static constexpr size_t kScratchSpec = kScratchXlist + (size_t)kNoAllocRays * sizeof(int);
+static constexpr size_t kScratchBytes =
+    kScratchSpec + (size_t)kMaxCam * kSpecBlocks * kMaxFreq * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch[kMaxScratchDevices];
+
+// The device address of `scratch` on `device`, which must be current:
+// resolved once per device.
+static hipError_t scratch_address(int device, void **out)
+{
+    if (device < 0 || device >= kMaxScratchDevices)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        if (!p)
+            return hipErrorInvalidSymbol;
+        g_scratch[device] = p;
+    }
+    *out = g_scratch[device];
+    return hipSuccess;
+}
 
 static void pin(rt_ctx *c, void *p)
 {
@@ -854,7 +896,7 @@ static hipError_t run_launches(rt_ctx *c, const Params &P, TraceArgs A, int tota
         if (A.nsaved > 0)
             hipLaunchKernelGGL(approach_kernel<INT>, dim3((unsigned)((A.nsaved + 63) / 64)),
                                dim3(64), 0, c->stream, P, A);
-        hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue);
+        hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, c->stream, A.queue);
         if ((e = hipGetLastError()) != hipSuccess)
             return e;
         if (mark && (e = hipEventRecord(c->evA, c->stream)) != hipSuccess)
@@ -932,21 +974,37 @@ static int ensure_state(rt_ctx *c, long long slots, RayState *out)
 // times longer than escaping rays -- are claimed first, so the launch does
 // not end waiting on a few long rays that started late.  Built on the host
 // once per image size and tile, kept in HBM (4 bytes per pixel).
-static int ensure_order(rt_ctx *c, int W, int H, int T, double sx, double sy, const int **out)
+static const int *find_order(rt_ctx *c, const long long key[5])
+{
+    for (auto &t : c->orders) {
+        if (memcmp(key, t.key, 5 * sizeof(long long)) == 0) {
+            t.used = ++c->use_clock;
+            if (c->capturing)
+                pin(c, t.d);
+            return t.d;
+        }
+    }
+    return nullptr;
+}
+
+static void order_key(int W, int H, int T, double sx, double sy, long long key[5])
 {
     long long bx, by;
     memcpy(&bx, &sx, sizeof bx);
     memcpy(&by, &sy, sizeof by);
-    const long long key[5] = {W, H, T, bx, by};
-    for (auto &t : c->orders) {
-        if (memcmp(key, t.key, sizeof key) == 0) {
-            t.used = ++c->use_clock;
-            if (c->capturing)
-                pin(c, t.d);
-            *out = t.d;
-            return RT_OK;
-        }
-    }
+    key[0] = W;
+    key[1] = H;
+    key[2] = T;
+    key[3] = bx;
+    key[4] = by;
+}
+
+static int ensure_order(rt_ctx *c, int W, int H, int T, double sx, double sy, const int **out)
+{
+    long long key[5];
+    order_key(W, H, T, sx, sy, key);
+    if ((*out = find_order(c, key)))
+        return RT_OK;
     if (c->capturing)
         return fail(c, RT_E_STATE, "the claim order for a new image shape is built by an eager "
                                    "render; render once before capturing");
@@ -1020,9 +1078,13 @@ static int ensure_order(rt_ctx *c, int W, int H, int T, double sx, double sy, co
     return RT_OK;
 }
 
+// noalloc (rt_render_device_noalloc): the work queue, the FP64 re-trace's
+// list and the spectrum's partial sums in the device's `scratch`; one fused
+// launch of at most kNoAllocRays work items; a centre-out claim order only
+// if one was built before for the image shape, tiled scan order otherwise.
 static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *freqs, int nf,
                        double *d_image, double *d_dimage, rt_ray_counters *d_counters,
-                       double *d_spectrum, bool grad)
+                       double *d_spectrum, bool grad, bool noalloc = false)
 {
     if (!c)
         return RT_E_INVALID;
@@ -1038,16 +1100,25 @@ static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *fr
                                      "emission model (RT_EMIS_THSYNCH) only");
     if ((rc = select_device(c)))
         return rc;
+    char *const scr = (char *)c->d_scratch;
     A.prims = c->d_prims;
-    A.queue = c->d_queue;
+    A.queue = noalloc ? (unsigned int *)(scr + kScratchQueue) : c->d_queue;
     A.refill = std::min(64, std::max(1, c->cfg.refill));
     A.tile = std::max(1, c->cfg.tile);
     A.order = nullptr;
     if (c->cfg.order != RT_ORDER_SCAN) {
-        rc = ensure_order(c, P.width, P.height, A.tile, P.cam_x / P.width, P.cam_y / P.height,
-                          &A.order);
-        if (rc)
-            return rc;
+        if (noalloc) {
+            long long key[5];
+            order_key(P.width, P.height, A.tile, P.cam_x / P.width, P.cam_y / P.height, key);
+            A.order = find_order(c, key);
+        } else {
+            rc = ensure_order(c, P.width, P.height, A.tile, P.cam_x / P.width,
+                              P.cam_y / P.height, &A.order);
+            if (rc)
+                return rc;
+        }
+    }
+    if (A.order) {
         A.ntx = 0;
         A.per_cam = P.npix;
     } else if (A.tile > 1) {
@@ -1062,20 +1133,28 @@ static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *fr
         A.per_cam = P.npix;
     }
     const int total = ncam * A.per_cam;
+    if (noalloc && total > kNoAllocRays)
+        return fail(c, RT_E_INVALID, "rt_render_device_noalloc: %d work items (n_cam x W x H, "
+                    "in whole claim tiles) exceed its scratch's %d", total, kNoAllocRays);
     A.image = d_image;
     A.dimage = grad ? d_dimage : nullptr;
     A.counters = d_counters;
     A.st = RayState{};
-    if (!c->d_xlist) {
-        if (c->capturing)
-            return fail(c, RT_E_STATE, "the FP64 re-trace's ray list is allocated by an eager "
-                                       "render; render once before capturing");
-        HIPCHK(c, hipMalloc((void **)&c->d_xlist, (size_t)kMaxChunk * sizeof(int)));
+    if (noalloc) {
+        A.xlist = (int *)(scr + kScratchXlist);
+    } else {
+        if (!c->d_xlist) {
+            if (c->capturing)
+                return fail(c, RT_E_STATE, "the FP64 re-trace's ray list is allocated by an "
+                                           "eager render; render once before capturing");
+            HIPCHK(c, hipMalloc((void **)&c->d_xlist, (size_t)kMaxChunk * sizeof(int)));
+        }
+        A.xlist = c->d_xlist;
     }
-    A.xlist = c->d_xlist;
-    A.xcount = c->d_queue + 1;
-    const int pct = c->cfg.approach != 0 ? c->cfg.approach
-                                         : (total >= kApproachRays ? 100 : RT_APPROACH_NONE);
+    A.xcount = A.queue + 1;
+    const int pct = noalloc ? RT_APPROACH_NONE
+                    : c->cfg.approach != 0 ? c->cfg.approach
+                                           : (total >= kApproachRays ? 100 : RT_APPROACH_NONE);
     if (pct > 0 &&
         (rc = ensure_state(c, (std::min<long long>(total, kMaxChunk) * pct + 99) / 100, &A.st)))
         return rc;
@@ -1096,10 +1175,11 @@ static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *fr
             nf == 1 ? spectrum_partial<1>
                     : (nf <= 4 ? spectrum_partial<4>
                                : (nf <= 8 ? spectrum_partial<8> : spectrum_partial<16>));
+        double *spec_part = noalloc ? (double *)(scr + kScratchSpec) : c->d_spec_part;
         hipLaunchKernelGGL(part, dim3((unsigned)nb, (unsigned)ncam), dim3(256), 0, c->stream,
-                           (const double *)d_image, P.npix, nf, c->d_spec_part);
+                           (const double *)d_image, P.npix, nf, spec_part);
         hipLaunchKernelGGL(spectrum_final, dim3((unsigned)ncam), dim3(64), 0, c->stream,
-                           (const double *)c->d_spec_part, nb, nf, d_spectrum);
+                           (const double *)spec_part, nb, nf, d_spectrum);
         HIPCHK(c, hipGetLastError());
     }
     if (!c->capturing)
@@ -1189,6 +1269,11 @@ rt_ctx *rt_create(int device)
         return nullptr;
     }
     c->num_cu = prop.multiProcessorCount;
+    if ((e = scratch_address(device, &c->d_scratch)) != hipSuccess) {
+        fail(nullptr, RT_E_HIP, "the no-allocation scratch's address: %s", hipGetErrorString(e));
+        rt_destroy(c);
+        return nullptr;
+    }
     if ((e = hipMalloc((void **)&c->d_spec_part,
                        (size_t)kMaxCam * kSpecBlocks * kMaxFreq * sizeof(double))) != hipSuccess ||
         (e = hipMalloc((void **)&c->d_spec, (size_t)kMaxCam * kMaxFreq * sizeof(double))) !=
@@ -1333,6 +1418,18 @@ int rt_render_device(rt_ctx *c, const double *incl, int ncam, const double *freq
         return RT_E_INVALID;
     return render_impl(c, incl, ncam, freqs, nf, d_image, d_dimage, d_counters, d_spectrum,
                        d_dimage != nullptr);
+}
+
+int rt_render_device_noalloc(rt_ctx *c, const double *incl, int ncam, const double *freqs,
+                             int nf, double *d_image, double *d_dimage,
+                             rt_ray_counters *d_counters, double *d_spectrum)
+{
+    if (!c)
+        return RT_E_INVALID;
+    if (!c->d_scratch)
+        return fail(c, RT_E_STATE, "no scratch address for device %d", c->device);
+    return render_impl(c, incl, ncam, freqs, nf, d_image, d_dimage, d_counters, d_spectrum,
+                       d_dimage != nullptr, true);
 }
 
 int rt_synchronize(rt_ctx *c)

@@ -63,6 +63,7 @@ ORDER_SCAN = 2
 # rt_launch_config.approach: percent of the rays whose camera-to-grid leg runs
 # in a kernel of its own (1..100), or none
 APPROACH_NONE = -1
+NOALLOC_MAX_RAYS = 1 << 22   # RT_NOALLOC_MAX_RAYS
 
 
 class RtError(RuntimeError):
@@ -81,6 +82,8 @@ SIGNATURES = [
     ("rt_render_grad", ctypes.c_int, [_vp, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp]),
     ("rt_render_device", ctypes.c_int, [_vp, _dp, ctypes.c_int, _dp, ctypes.c_int, _vp, _vp, _vp,
                                          _vp]),
+    ("rt_render_device_noalloc", ctypes.c_int, [_vp, _dp, ctypes.c_int, _dp, ctypes.c_int, _vp,
+                                                 _vp, _vp, _vp]),
     ("rt_synchronize", ctypes.c_int, [_vp]),
     ("rt_last_kernel_ms", ctypes.c_double, [_vp]),
     ("rt_last_kernel_times", ctypes.c_int, [_vp, _dp]),
@@ -238,6 +241,16 @@ class Tracer:
         fr = np.ascontiguousarray(np.atleast_1d(freqs), float)
         self._check(self.lib.rt_render_device(self.ctx, _d(incl), incl.size, _d(fr), fr.size,
                                               d_image, d_dimage, d_counters, d_spectrum))
+
+    def render_device_noalloc(self, inclinations, freqs, d_image, d_dimage=None,
+                              d_counters=None, d_spectrum=None):
+        """render_device without any device allocation (the library's static
+        scratch, at most NOALLOC_MAX_RAYS work items); the same outputs."""
+        incl = np.ascontiguousarray(np.atleast_1d(inclinations), float)
+        fr = np.ascontiguousarray(np.atleast_1d(freqs), float)
+        self._check(self.lib.rt_render_device_noalloc(self.ctx, _d(incl), incl.size, _d(fr),
+                                                      fr.size, d_image, d_dimage, d_counters,
+                                                      d_spectrum))
 
     def synchronize(self):
         self._check(self.lib.rt_synchronize(self.ctx))
