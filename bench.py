@@ -62,6 +62,15 @@ def host_cores():
     return max(1, n)
 
 
+def _json(obj):
+    """One JSON line; numpy scalars as Python numbers."""
+    def scalar(o):
+        if isinstance(o, np.generic):
+            return o.item()
+        raise TypeError("not JSON serializable: %r" % type(o))
+    return json.dumps(obj, default=scalar)
+
+
 def _dump(*shape):
     from raptor_amd import torus
     with warnings.catch_warnings():
@@ -139,9 +148,10 @@ def _counters(tr, incl, freqs, grad=False):
 
 def _rooflines(cnt, nf, kms, per, grid_step=None, grad=False, kernel="trace kernel"):
     """Roofline of the render's kernels together and, given the rays' grid
-    entry steps (the oracle's counters; per ray, or a sample: then the
-    approach share of the steps is scaled to the image), of the trace kernel
-    and the approach kernel on their own."""
+    entry steps (the oracle's counters) with the kernels' own steps of the
+    same rays (per ray, or a sample: then the approach share of the steps is
+    scaled to the image), of the trace kernel and the approach kernel on
+    their own."""
     from raptor_amd import flops
     f64, f32 = flops.image_flops_split(cnt[:, 0], cnt[:, 1], nf, grad=grad)
     out = {"render": _roof(f64, f32, kms, "all kernels of the render"),
@@ -220,7 +230,7 @@ def bench_c1(args):
         "config": dict(WORKLOAD, image="%dx%d" % (W, W), emission=args.emission),
         "launch": dict(launch, approach_effective=100 if split else -1),
         "images_per_s": 1e3 / ms_per_step, "flux_jy": spec * jy,
-        "spectrum_equals_image_sum": abs(spec / img.sum() - 1) < 1e-12,
+        "spectrum_equals_image_sum": bool(abs(spec / img.sum() - 1) < 1e-12),
         "kernels": per, "kernel_ms": kms,
     }
 
@@ -253,7 +263,7 @@ def bench_c1(args):
         agree = np.where(op >= 0, (st > op) & (st <= ocnt[:, 0]), st == ocnt[:, 0])
         result["parity"]["steps_agree_oracle_frac"] = float(np.mean(agree))
         result["parity"]["opaque_frac"] = float(np.mean(op >= 0))
-        grid_step = (ocnt[:, 4], ocnt[:, 0])
+        grid_step = (ocnt[:, 4], cnt[pix, 0])
 
     roof = _rooflines(cnt, 1, kms, per, grid_step, kernel=trace_name) if thermal else None
     traffic = issue = None
@@ -460,7 +470,7 @@ def bench_3d(args, which):
                       "%.1f s wall" % (stride, pix.size, c1 - c0)}
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
         out["parity"] = _parity(img[pix], ora0[:, 0], gate0[:, 0])
-        grid_step = (ocnt[:, 4], ocnt[:, 0])
+        grid_step = (ocnt[:, 4], cnt[pix, 0])
     split = npix >= 1 << 20
     out["image"]["roofline"] = _rooflines(
         cnt, 1, ki, peri if split else None, grid_step if split else None,
@@ -535,10 +545,10 @@ def main():
         sys.exit(2)
 
     if args.workload == "c2":
-        print(json.dumps(bench_c2(args)), flush=True)
+        print(_json(bench_c2(args)), flush=True)
         return
     if args.workload in ("c3", "c4"):
-        print(json.dumps(bench_3d(args, args.workload)), flush=True)
+        print(_json(bench_3d(args, args.workload)), flush=True)
         return
     result = bench_c1(args)
     if not args.no_configs and args.size == 1024 and args.emission == "thsynch":
@@ -549,7 +559,7 @@ def main():
                              "c4": bench_3d(sub, "c4")}
         result["configs_note"] = ("each workload's own ms, parity against the FP64 oracle on "
                                   "every 64th pixel and roofline; not part of value")
-    print(json.dumps(result), flush=True)
+    print(_json(result), flush=True)
 
 
 if __name__ == "__main__":

@@ -9,8 +9,9 @@
  *                                                      (functions.h:24-25, raptor_harm_model.c:190-320)
  * with one HIP implementation on one GPU.  Everything the reference passes
  * through globals (parameters.h:58-189, constants.h:32-35) is passed here
- * explicitly; there is no global state, and one context serves one host
- * thread at a time.
+ * explicitly; there is no global state (beyond the device array of
+ * rt_render_device_noalloc), and one context serves one host thread at a
+ * time.
  *
  * Conventions
  *  - Every function returns RT_OK (0) or a negative RT_E* code; the message
@@ -87,7 +88,10 @@ typedef struct {
 #define RT_EMIS_KAPPA 2
 
 /* Per-ray counters written when requested: RK2 steps taken and samples for
- * which the transfer was evaluated (inside the grid and past the sigma cut). */
+ * which the transfer was evaluated (inside the grid and past the sigma cut).
+ * A ray ends once its optical depth has passed ln 1000 at every frequency
+ * (no later sample reaches its pixel, radiative_transfer.c:90-95), so such a
+ * ray counts fewer steps than the reference's loop takes. */
 typedef struct {
     int32_t steps;
     int32_t samples;
@@ -224,8 +228,11 @@ int rt_get_launch_config(const rt_ctx *ctx, rt_launch_config *cfg);
  * rt_capture_end into one HIP graph (no other rt_* call that allocates,
  * copies or synchronises is allowed in between).  rt_graph_launch replays it
  * on the context's stream without synchronising; rt_last_kernel_ms then
- * times the whole replay.  No counterpart in the reference, whose pixel loop
- * is one blocking call (core.c:97-114). */
+ * times the whole replay.  A graph renders the primitives that were loaded
+ * when it was captured: rt_load_prims keeps that buffer alive, beside the
+ * new one, until the last graph using it is destroyed.  Destroy a context's
+ * graphs before the context.  No counterpart in the reference, whose pixel
+ * loop is one blocking call (core.c:97-114). */
 typedef struct rt_graph rt_graph;
 int rt_capture_begin(rt_ctx *ctx);
 int rt_capture_end(rt_ctx *ctx, rt_graph **graph);

@@ -636,6 +636,15 @@ struct rt_ctx {
     rt_params params{};
     rt_grid grid{};
     float4 *d_prims = nullptr;
+    // Live graphs that launch with d_prims.  rt_load_prims keeps a buffer
+    // that graphs still use in retired_prims (they go on rendering the dump
+    // they were captured with) until the last of them is destroyed.
+    int prims_refs = 0;
+    struct RetiredPrims {
+        void *d;
+        int refs;
+    };
+    std::vector<RetiredPrims> retired_prims;
     unsigned int *d_queue = nullptr;
     // Centre-out claim orders, one per image shape (W, H, tile, pixel size
     // bits); beyond kMaxOrderTables the least recently used one is freed.
@@ -1102,6 +1111,8 @@ static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *fr
         return rc;
     char *const scr = (char *)c->d_scratch;
     A.prims = c->d_prims;
+    if (c->capturing)
+        pin(c, c->d_prims);
     A.queue = noalloc ? (unsigned int *)(scr + kScratchQueue) : c->d_queue;
     A.refill = std::min(64, std::max(1, c->cfg.refill));
     A.tile = std::max(1, c->cfg.tile);
@@ -1304,6 +1315,8 @@ void rt_destroy(rt_ctx *c)
     if (c->stream)
         (void)hipStreamSynchronize(c->stream);
     (void)hipFree(c->d_prims);
+    for (auto &r : c->retired_prims)
+        (void)hipFree(r.d);
     (void)hipFree(c->d_queue);
     (void)hipFree(c->d_xlist);
     (void)hipFree(c->d_spec_part);
@@ -1373,8 +1386,18 @@ int rt_load_prims(rt_ctx *c, const rt_grid *g, const float *h_prims)
         return rc;
     size_t ncell = (size_t)g->n1 * g->n2 * g->n3;
     if (c->d_prims) {
-        HIPCHK(c, hipFree(c->d_prims));
+        if (c->prims_refs > 0) {
+            try {
+                c->retired_prims.push_back({c->d_prims, c->prims_refs});
+            } catch (const std::bad_alloc &) {
+                return fail(c, RT_E_NOMEM, "out of host memory");
+            }
+        } else {
+            HIPCHK(c, hipStreamSynchronize(c->stream));   // renders still reading it
+            HIPCHK(c, hipFree(c->d_prims));
+        }
         c->d_prims = nullptr;
+        c->prims_refs = 0;
     }
     c->have_grid = false;
     float *d_soa = nullptr;
@@ -1533,6 +1556,8 @@ int rt_capture_end(rt_ctx *c, rt_graph **out)
     for (auto &b : c->states)
         if (pinned(b.d))
             b.refs++;
+    if (c->d_prims && pinned(c->d_prims))
+        c->prims_refs++;
     *out = G;
     return RT_OK;
 }
@@ -1578,6 +1603,19 @@ int rt_graph_destroy(rt_ctx *c, rt_graph *g)
     for (auto &b : c->states)
         if (b.refs > 0 && pinned(b.d))
             b.refs--;
+    if (c->prims_refs > 0 && c->d_prims && pinned(c->d_prims))
+        c->prims_refs--;
+    for (size_t i = 0; i < c->retired_prims.size();) {
+        auto &r = c->retired_prims[i];
+        if (r.refs > 0 && pinned(r.d))
+            r.refs--;
+        if (r.refs == 0) {
+            (void)hipFree(r.d);
+            c->retired_prims.erase(c->retired_prims.begin() + (long)i);
+        } else {
+            i++;
+        }
+    }
     // older state buffers that no graph uses any more (the stream is idle)
     for (size_t i = 0; i + 1 < c->states.size();) {
         if (c->states[i].refs == 0) {

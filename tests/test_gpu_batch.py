@@ -278,6 +278,56 @@ def test_graph_keeps_its_claim_order_after_other_shapes(tracer):
         tracer.free(d1)
 
 
+def test_graph_keeps_its_primitives_after_a_new_dump(tracer, dump128):
+    """rt_load_prims while graphs still launch with the old primitives: the
+    graphs go on rendering the dump they were captured with (the buffer is
+    kept until the last of them is destroyed), eager renders use the new
+    dump, and a graph captured after the load renders the new one."""
+    import dataclasses
+    W = 32
+    _params(tracer, W)
+    n = W * W
+    other = dataclasses.replace(dump128, prims=dump128.prims.copy())
+    other.prims[0] *= 1.5   # denser: a different image
+    d1, d2 = tracer.alloc(n * 8), tracer.alloc(n * 8)
+    graphs = []
+    try:
+        tracer.load_dump(dump128)
+        e_old, _ = tracer.render([INCL], [FREQ])
+        for _ in range(2):   # two graphs on the old buffer
+            tracer.capture_begin()
+            tracer.render_device([INCL], [FREQ], d1)
+            graphs.append(tracer.capture_end())
+        tracer.load_dump(other)
+        e_new, _ = tracer.render([INCL], [FREQ])
+        assert not np.array_equal(e_old, e_new)
+        tracer.capture_begin()
+        tracer.render_device([INCL], [FREQ], d2)
+        graphs.append(tracer.capture_end())
+        for k, g in enumerate(graphs):
+            g.launch()
+            tracer.synchronize()
+            want = e_new if k == 2 else e_old
+            got = tracer.d2h(d2 if k == 2 else d1, (n,), np.float64)
+            assert np.array_equal(got, want[0, :, 0]), k
+        graphs.pop(0).destroy()   # one graph still holds the old buffer
+        graphs[0].launch()
+        tracer.synchronize()
+        assert np.array_equal(tracer.d2h(d1, (n,), np.float64), e_old[0, :, 0])
+        tracer.load_dump(dump128)   # the old dump again, in a new buffer
+        img, _ = tracer.render([INCL], [FREQ])
+        assert np.array_equal(img, e_old)
+        graphs[1].launch()
+        tracer.synchronize()
+        assert np.array_equal(tracer.d2h(d2, (n,), np.float64), e_new[0, :, 0])
+    finally:
+        for g in graphs:
+            g.destroy()
+        tracer.free(d1)
+        tracer.free(d2)
+        tracer.load_dump(dump128)
+
+
 def test_calls_that_synchronise_are_refused_during_capture(tracer):
     _params(tracer, 8)
     tracer.capture_begin()
