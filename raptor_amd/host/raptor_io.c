@@ -177,6 +177,7 @@ static int read_ascii(FILE *fp, rio_dump *d, char *err, size_t errlen)
     if (!d->prims)
         return seterr(err, errlen, "out of memory for %zu cells", ncell);
     double rec[34];
+    double dMact = 0., Ladv = 0.;
     for (size_t k = 0; k < ncell; k++) {
         for (int q = 0; q < 34; q++)
             if (fscanf(fp, "%lf", &rec[q]) != 1) {
@@ -184,11 +185,33 @@ static int read_ascii(FILE *fp, rio_dump *d, char *err, size_t errlen)
                 d->prims = NULL;
                 return seterr(err, errlen, "HARM dump: record %zu of %zu incomplete", k, ncell);
             }
-        /* record k = i*n2 + j; primitives are columns 4..11 */
+        /* record k = i*n2 + j: x1 x2 r h, the primitives (4..11), divb,
+         * U^mu (13..16), U_mu (17..20), B^mu, B_mu, 4 bounds, gdet (33) */
         for (int v = 0; v < 8; v++)
             d->prims[v * ncell + k] = (float)rec[4 + v];
+        const size_t i = k / (size_t)n2;
+        const double gdet = rec[33], ucon1 = rec[14], ucov0 = rec[17];
+        if (i <= 20)
+            dMact += gdet * rec[4] * ucon1;
+        if (i >= 20 && i < 40)
+            Ladv += gdet * rec[5] * ucon1 * ucov0;
     }
+    /* over phi (dx3 = 2 pi for the 2D dump) and theta, per 21 zones */
+    d->dMact = dMact * (2. * M_PI) * g->dx2 / 21.;
+    d->Ladv = Ladv * (2. * M_PI) * g->dx2 / 21.;
+    d->have_fluxes = 1;
     return 0;
+}
+
+void rio_print_fluxes(FILE *fp, const rio_dump *d, double M_unit, double MBH)
+{
+    if (d->have_fluxes) {
+        const double c = 2.99792458e10, G = 6.6742e-8, msun = 1.989e33, year = 31536000.;
+        const double T_unit = G * MBH / (c * c * c);
+        fprintf(fp, "dMact: %g Macc: %e [Msun/year], Ladv: %g\n", d->dMact,
+                d->dMact * M_unit / T_unit / (msun / year), d->Ladv);
+    }
+    fprintf(fp, "Done reading data\n\n");
 }
 
 static int read_bin(FILE *fp, rio_dump *d, char *err, size_t errlen)

@@ -5,6 +5,8 @@
 #ifndef RAPTOR_IO_H
 #define RAPTOR_IO_H
 
+#include <stdio.h>
+
 #include "raptor_rt.h"
 
 #define RIO_MAX_FREQ RT_MAX_FREQ
@@ -36,6 +38,11 @@ typedef struct {
     rt_grid grid;
     double gam, Rin, Rout, t;
     float *prims;                 /* 8 * n1 * n2 * n3 */
+    /* accretion diagnostics of an ASCII dump (raptor_harm_model.c:171-180),
+     * code units: rest-mass flux through the inner 21 radial zones and the
+     * advected energy flux through zones 20..39, per 21 zones */
+    int have_fluxes;
+    double dMact, Ladv;
 } rio_dump;
 
 /* model.in (core.c:22-34): 11 records "NAME (unit) value" in fixed order.
@@ -57,6 +64,11 @@ int rio_parse_args(int argc, char **argv, rio_model *m, char *err, size_t errlen
  * primitive file (magic "RPTRPRIM", any N3); the format is detected from the
  * first bytes. */
 int rio_read_dump(const char *path, rio_dump *d, char *err, size_t errlen);
+/* The reference's line after reading the dump (raptor_harm_model.c:181-182):
+ * dMact, the accretion rate in Msun/year (dMact M_unit / T_unit with
+ * T_unit = G MBH / c^3) and Ladv, then "Done reading data".  Prints nothing
+ * but the last line for dumps without the fluxes. */
+void rio_print_fluxes(FILE *fp, const rio_dump *d, double M_unit, double MBH);
 void rio_free_dump(rio_dump *d);
 
 /* JANSKY_FACTOR (utilities.c:32-42). */

@@ -54,6 +54,7 @@ int main(int argc, char **argv)
     }
     fprintf(stderr, "Read %s: %d x %d x %d cells (%.2f s)\n", m.GRMHD_FILE, d.grid.n1,
             d.grid.n2, d.grid.n3, now_s() - t0);
+    rio_print_fluxes(stderr, &d, m.M_UNIT, m.MBH);
 
     rt_ctx *ctx = rt_create(m.device);
     if (!ctx) {

@@ -71,6 +71,41 @@ def test_model_in_and_dump_errors(driver, tmp_path):
     assert r.returncode == 1 and "incomplete" in r.stderr
 
 
+def test_driver_prints_accretion_fluxes(driver, tmp_path):
+    """The line init_model prints after reading an ASCII dump
+    (raptor_harm_model.c:171-182): dMact, the sum of gdet rho U^1 over radial
+    zones 0..20, and Ladv, of gdet u U^1 U_0 over zones 20..39, each times
+    dx3 dx2 (dx3 = 2 pi) and over 21; Macc = dMact M_unit / T_unit in
+    Msun/year, T_unit = G MBH / c^3.  Printed before any GPU call, so checked
+    here on the CPU, on a dump with random radial velocities (the torus
+    itself has U^1 = 0)."""
+    import re
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        d = torus.fishbone_moncrief(48, 24)
+    rng = np.random.default_rng(3)
+    d.extras["ucon"][1] = rng.uniform(-0.5, 0.5, d.extras["ucon"][1].shape)
+    d.extras["ucov"][0] = rng.uniform(-1.2, -0.8, d.extras["ucov"][0].shape)
+    harm.write_harm_ascii(str(tmp_path / "dump.txt"), d)
+    (tmp_path / "model.in").write_text(MODEL_IN.format(W=4, H=4))
+    r = _run(driver, ["model.in", "dump.txt", "%g" % M_UNIT, "60", "%g" % R_HIGH, "1", "0"],
+             tmp_path)
+    m = re.search(r"dMact: (\S+) Macc: (\S+) \[Msun/year\], Ladv: (\S+)\n", r.stderr)
+    assert m, r.stderr
+    assert "Done reading data" in r.stderr[m.end():]
+    g, ex = d.grid, d.extras
+    rho, uu = d.prims[0, :, :, 0], d.prims[1, :, :, 0]
+    flux = ex["gdet"] * ex["ucon"][1]
+    dM = (flux[:21] * rho[:21]).sum() * 2 * np.pi * g.dx2 / 21
+    L = (flux[20:40] * uu[20:40] * ex["ucov"][0][20:40]).sum() * 2 * np.pi * g.dx2 / 21
+    T_unit = 6.6742e-8 * 8.9505e39 / 2.99792458e10 ** 3
+    macc = dM * M_UNIT / T_unit / (1.989e33 / 31536000.0)
+    assert abs(dM) > 0 and abs(L) > 0
+    assert float(m.group(1)) == pytest.approx(dM, rel=1e-5)
+    assert float(m.group(2)) == pytest.approx(macc, rel=1e-6)
+    assert float(m.group(3)) == pytest.approx(L, rel=1e-5)
+
+
 def _expected_files(tmp_path, image, W, H, freqs, incl, dump_name, jf, t=0):
     """The reference's writers (core.c:163-198, utilities.c:51-61) applied to
     an image [W*H][nf]."""
