@@ -148,7 +148,7 @@ int rt_render_device(rt_ctx *ctx, const double *inclinations, int n_cam,
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  Same arguments and outputs, bit for bit.  The work queue,
  * the FP64 re-trace's ray list and the spectrum's partial sums live in one
- * static device array of the library (18 MiB per GPU, held until the
+ * static device array of the library (34 MiB per GPU, held until the
  * process ends) instead of buffers the library allocates; hence at most
  * RT_NOALLOC_MAX_RAYS work items per call (n_cam x W x H, in whole claim
  * tiles), RT_E_INVALID beyond.  It traces every ray from the camera in one
@@ -199,7 +199,7 @@ double rt_jansky_factor(const rt_ctx *ctx);
  * whose camera-to-grid leg runs first in a kernel of its own: 1..100
  * percent, RT_APPROACH_NONE for none, 0 = library default (all rays for
  * launches of at least 2^20 rays, none below).  Those rays are
- * saved where they reach the grid (120 bytes each, at most 2^23 saved rays
+ * saved where they reach the grid (128 bytes each, at most 2^23 saved rays
  * per launch pair, larger batches in chunks) and resumed first by the
  * persistent kernel, whose waves then do no camera-to-grid steps beside
  * rays inside the grid; it traces the other rays from the camera, and

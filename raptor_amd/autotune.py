@@ -60,10 +60,28 @@ def apply_cached(tracer, n_cam, n_freq, grad=False, dim3=False, path=CACHE):
     return entry["config"]
 
 
+def _time(tracer, cfg, incl, fr, d_img, d_dimg, reps):
+    """Median kernel ms of `reps` renders with launch configuration cfg (after
+    one untimed render that builds whatever the shape needs)."""
+    tracer.set_launch_config(**cfg)
+    tracer.render_device(incl, fr, d_img, d_dimg)
+    tracer.synchronize()
+    ms = []
+    for _ in range(reps):
+        tracer.render_device(incl, fr, d_img, d_dimg)
+        tracer.synchronize()
+        ms.append(tracer.last_kernel_ms())
+    return float(np.median(ms))
+
+
 def tune(tracer, inclinations, freqs, grad=False, dim3=False, space=None, reps=2,
-         path=CACHE):
-    """Time every configuration of `space` on this workload; apply and cache
-    the fastest.  Returns {"config", "ms", "table"}."""
+         path=CACHE, refine=4, refine_reps=5):
+    """Time every configuration of `space` on this workload, then the `refine`
+    fastest again with `refine_reps` renders each (one sweep's minimum over
+    many shapes is biased towards a lucky timing: differences between the
+    leading shapes are a few percent, like the run-to-run noise); apply and
+    cache the fastest of those.  Returns {"config", "ms", "table",
+    "refined"}."""
     space = dict(DEFAULT_SPACE, **(space or {}))
     incl = np.atleast_1d(inclinations)
     fr = np.atleast_1d(freqs)
@@ -75,22 +93,17 @@ def tune(tracer, inclinations, freqs, grad=False, dim3=False, space=None, reps=2
         keys = sorted(space)
         for values in itertools.product(*(space[k] for k in keys)):
             cfg = dict(zip(keys, values))
-            tracer.set_launch_config(**cfg)
-            tracer.render_device(incl, fr, d_img, d_dimg)
-            tracer.synchronize()
-            ms = []
-            for _ in range(reps):
-                tracer.render_device(incl, fr, d_img, d_dimg)
-                tracer.synchronize()
-                ms.append(tracer.last_kernel_ms())
-            table.append((cfg, float(np.median(ms))))
+            table.append((cfg, _time(tracer, cfg, incl, fr, d_img, d_dimg, reps)))
+        lead = sorted(table, key=lambda t: t[1])[:max(1, refine)]
+        refined = [(cfg, _time(tracer, cfg, incl, fr, d_img, d_dimg, refine_reps))
+                   for cfg, _ in lead]
     finally:
         tracer.free(d_img)
         if d_dimg:
             tracer.free(d_dimg)
-    best_cfg, best_ms = min(table, key=lambda t: t[1])
+    best_cfg, best_ms = min(refined, key=lambda t: t[1])
     tracer.set_launch_config(**best_cfg)
     if path:
         _store(path, workload_key(tracer, incl.size, fr.size, grad, dim3),
                {"config": best_cfg, "ms": best_ms})
-    return {"config": best_cfg, "ms": best_ms, "table": table}
+    return {"config": best_cfg, "ms": best_ms, "table": table, "refined": refined}

@@ -37,26 +37,28 @@ namespace {
 
 // Rays that have reached the grid, saved by the approach kernel for the trace
 // kernel to resume, each as it was before the step that took it into the
-// grid.  Slots in blocks of 64 (one wave's rays), each block a structure of
-// arrays: 13 doubles [13][64] (X^mu, k^mu, r, sin/cos theta, sin/cos 2 pi X2)
-// then 4 ints [4][64] (steps, turns, ray, flags), so a wave's store of one
-// field is 512 contiguous bytes and every field sits at a fixed offset.
+// grid: one 128-byte record per ray, one cache line, written and read whole
+// as eight 16-byte vectors, so that a lane's refill reads its own lines
+// only (with the fields of 16 neighbouring rays in one line, refills of a
+// few lanes on different CUs fetched each line several times).
 // ray < 0: nothing left to trace.  flags: theta-dot sign + 1 | (beta sign +
 // 1) << 2.
+struct alignas(16) SavedRay {
+    double y[8];    // X^mu, k^mu
+    double pt[5];   // r, sin theta, cos theta, sin 2 pi X2, cos 2 pi X2
+    int steps, turns, ray, flags;
+    int pad[2];
+};
+static_assert(sizeof(SavedRay) == 128, "one cache line per saved ray");
 struct RayState {
     char *d;
     int n;   // slots (a multiple of 64); 0: no saved state
 };
-static constexpr int kStateBlockBytes = 64 * (13 * 8 + 4 * 4);
-static constexpr size_t kStateBytes = kStateBlockBytes / 64;
+static constexpr size_t kStateBytes = sizeof(SavedRay);
 
-__device__ inline double *state_f64(const RayState &S, int i)
+__device__ inline SavedRay *state_rec(const RayState &S, int i)
 {
-    return (double *)(S.d + (size_t)(i >> 6) * kStateBlockBytes) + (i & 63);
-}
-__device__ inline int *state_i32(const RayState &S, int i)
-{
-    return (int *)(S.d + (size_t)(i >> 6) * kStateBlockBytes + 13 * 64 * 8) + (i & 63);
+    return (SavedRay *)S.d + i;
 }
 
 struct TraceArgs {
@@ -73,9 +75,10 @@ struct TraceArgs {
     double *dimage;
     rt_ray_counters *counters;
     RayState st;        // the saved rays
-    // rays to trace again with the FP64 transfer (exact_kernel): their work
-    // items' indices within the launch, xcount of them
-    int *xlist;
+    // rays to trace again with the FP64 transfer (exact_kernel), xcount of
+    // them: per ray its work item's index within the launch and the mask of
+    // frequencies that had a sample in the gate window (Accum::gmask)
+    int2 *xlist;
     unsigned int *xcount;
 };
 
@@ -148,48 +151,47 @@ __device__ inline void write_empty(const Params &P, const TraceArgs &A, int ray,
 template <int NF, bool GRAD>
 __device__ inline void save_state(const RayState &S, int i, int ray, const Ray<NF, GRAD> &R)
 {
-    double *f = state_f64(S, i);
-    int *k = state_i32(S, i);
+    SavedRay v;
 #pragma unroll
     for (int q = 0; q < 8; q++)
-        f[q * 64] = R.y[q];
-    f[8 * 64] = R.pt.r;
-    f[9 * 64] = R.pt.s;
-    f[10 * 64] = R.pt.c;
-    f[11 * 64] = R.pt.s2;
-    f[12 * 64] = R.pt.c2;
-    k[0] = R.steps;
-    k[64] = R.turns;
-    k[128] = ray;
-    k[192] = (R.thetadot_sign + 1) | ((R.beta_sign + 1) << 2);
+        v.y[q] = R.y[q];
+    v.pt[0] = R.pt.r;
+    v.pt[1] = R.pt.s;
+    v.pt[2] = R.pt.c;
+    v.pt[3] = R.pt.s2;
+    v.pt[4] = R.pt.c2;
+    v.steps = R.steps;
+    v.turns = R.turns;
+    v.ray = ray;
+    v.flags = (R.thetadot_sign + 1) | ((R.beta_sign + 1) << 2);
+    v.pad[0] = v.pad[1] = 0;
+    *state_rec(S, i) = v;
 }
 
-__device__ inline void drop_state(const RayState &S, int i) { state_i32(S, i)[128] = -1; }
+__device__ inline void drop_state(const RayState &S, int i) { state_rec(S, i)->ray = -1; }
 
-__device__ inline int state_ray(const RayState &S, int i) { return state_i32(S, i)[128]; }
+__device__ inline int state_ray(const RayState &S, int i) { return state_rec(S, i)->ray; }
 
 template <int NF, bool GRAD>
 __device__ inline void load_state(const RayState &S, int i, Ray<NF, GRAD> &R)
 {
-    const double *f = state_f64(S, i);
-    const int *k = state_i32(S, i);
+    const SavedRay v = *state_rec(S, i);
 #pragma unroll
     for (int q = 0; q < 8; q++)
-        R.y[q] = f[q * 64];
-    R.pt.r = f[8 * 64];
-    R.pt.s = f[9 * 64];
-    R.pt.c = f[10 * 64];
-    R.pt.s2 = f[11 * 64];
-    R.pt.c2 = f[12 * 64];
+        R.y[q] = v.y[q];
+    R.pt.r = v.pt[0];
+    R.pt.s = v.pt[1];
+    R.pt.c = v.pt[2];
+    R.pt.s2 = v.pt[3];
+    R.pt.c2 = v.pt[4];
     // the fields in ray_init's order: the two set-ups are alternatives of one
     // branch, and the compiler pairs their stores in order when it merges
     // them, so a different order would pair different fields and leave the
     // counters in scratch memory behind a selected address
-    const int fl = k[192];
-    R.thetadot_sign = (fl & 3) - 1;
-    R.beta_sign = ((fl >> 2) & 3) - 1;
-    R.steps = k[0];
-    R.turns = k[64];
+    R.thetadot_sign = (v.flags & 3) - 1;
+    R.beta_sign = ((v.flags >> 2) & 3) - 1;
+    R.steps = v.steps;
+    R.turns = v.turns;
     R.samples = 0;
     R.terminate = false;
     R.acc.clear();
@@ -327,7 +329,7 @@ __device__ __forceinline__ void trace_rays(const Params &P, const TraceArgs &A)
                 } else {
                     write_ray<NF, GRAD, DIM3>(P, A, ray, R);
                     if (!kInline && needs_exact<EMIS>(R))
-                        A.xlist[atomicAdd(A.xcount, 1u)] = slot;
+                        A.xlist[atomicAdd(A.xcount, 1u)] = make_int2(slot, (int)R.acc.gmask);
                     active = false;
                     xm = false;
                 }
@@ -375,40 +377,46 @@ __global__ __launch_bounds__(64, 3) void trace_kernel_3w(const Params P, const T
 }
 
 // The rays a trace kernel listed (needs_exact) traced again from their start
-// (the camera, or the saved ray at the grid's edge) by exact_retrace: the
-// frequencies whose FP32 optical depth met the ln 1000 gate where a different
-// decision would show in the pixel get the FP64 transfer's outputs, written
-// over their FP32 ones.  A kernel of its own: its FP64 arithmetic needs more
-// registers than the trace kernel can give without losing its third wave (in
-// that kernel it cost 6-60 spilled VGPRs).  About 1 ray in 3000 at 1024^2
-// and 230 GHz; consecutive entries go to different waves, so that those few
-// rays run on different SIMDs.
-template <int NF, bool GRAD, bool DIM3, int INT, bool RESUME>
+// (the camera, or the saved ray at the grid's edge) by exact_retrace, one
+// frequency per lane: workgroup row blockIdx.y takes frequency f of every
+// listed ray whose gate mask has bit f (the frequencies that had a sample in
+// the gate window; for the others the FP32 outputs stand).  Where that
+// frequency's own share of the window exceeds kGateFlip it gets the FP64
+// transfer's outputs, written over its FP32 ones -- the same decision and
+// the same arithmetic as a launch of that frequency alone, whose trace
+// kernel traces the ray again itself.  A lane's second trace ends where its
+// frequency's optical depth has passed the gate, so that it runs from the
+// grid's edge to the crossing, not to the end of the ray at every
+// frequency.  A kernel of its own: its FP64 arithmetic needs more registers
+// than the multi-frequency trace kernels can give.  Consecutive entries go
+// to different waves, so that those few rays run on different SIMDs.
+template <bool GRAD, bool DIM3, int INT, bool RESUME>
 __global__ __launch_bounds__(64) void exact_kernel(const Params P, const TraceArgs A)
 {
     const unsigned n = *A.xcount;
+    const int f = blockIdx.y;
     const unsigned stride = gridDim.x * 64u;
     for (unsigned i = blockIdx.x + gridDim.x * threadIdx.x; i < n; i += stride) {
-        const int q = A.xlist[i];
+        const int2 e = A.xlist[i];
+        if (!((e.y >> f) & 1))
+            continue;
+        const int q = e.x;
         int ray;
         if (RESUME && q < A.nsaved)
             ray = state_ray(A.st, q);
         else if (!work_to_ray(P, A, A.w0 + q, ray))
             continue;
-        Ray<NF, GRAD> R;
+        Ray<1, GRAD> R;
         start_ray<RESUME>(P, A, ray, q, R);
-        Accum<NF, GRAD> x;
-        float wf[NF];
-        exact_retrace<NF, GRAD, DIM3, INT>(P, A.prims, R, x, wf);
-        const size_t o = (size_t)ray * P.nf;
-#pragma unroll
-        for (int f = 0; f < NF; f++) {
-            if (f < P.nf && wf[f] > kGateFlip) {
-                A.image[o + f] = (double)x.I[f];
-                if (GRAD) {
-                    A.dimage[(o + f) * 2 + 0] = x.dI[f][0];
-                    A.dimage[(o + f) * 2 + 1] = x.dI[f][1];
-                }
+        Accum<1, GRAD> x;
+        float wf[1];
+        exact_retrace<1, GRAD, DIM3, INT>(P, A.prims, R, x, wf, f);
+        if (wf[0] > kGateFlip) {
+            const size_t o = (size_t)ray * P.nf + f;
+            A.image[o] = (double)x.I[0];
+            if (GRAD) {
+                A.dimage[o * 2 + 0] = x.dI[0][0];
+                A.dimage[o * 2 + 1] = x.dI[0][1];
             }
         }
     }
@@ -670,7 +678,7 @@ struct rt_ctx {
     // work items of one launch to trace again with the FP64 transfer
     // (exact_kernel), kMaxChunk of them: made by the first eager render and
     // kept, so that a captured graph's launches always find it
-    int *d_xlist = nullptr;
+    int2 *d_xlist = nullptr;
     // spectrum: partial sums [camera][kSpecBlocks][frequency] and, for host
     // renders, the spectrum itself; made by rt_create
     double *d_spec_part = nullptr, *d_spec = nullptr;
@@ -721,7 +729,7 @@ static constexpr int kMaxChunk = 1 << 23;
 static constexpr int kNoAllocRays = RT_NOALLOC_MAX_RAYS;
 static constexpr size_t kScratchQueue = 0;     // 2 unsigned ints, 256 bytes reserved
 static constexpr size_t kScratchXlist = 256;
-static constexpr size_t kScratchSpec = kScratchXlist + (size_t)kNoAllocRays * sizeof(int);
+static constexpr size_t kScratchSpec = kScratchXlist + (size_t)kNoAllocRays * sizeof(int2);
 static constexpr size_t kScratchBytes =
     kScratchSpec + (size_t)kMaxCam * kSpecBlocks * kMaxFreq * sizeof(double);
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
@@ -857,29 +865,20 @@ static hipError_t dispatch_nf(rt_ctx *c, const Params &P, const TraceArgs &A)
     return launch_trace<16, GRAD, DIM3, INT, RESUME>(c, P, A);
 }
 
-// The FP64 re-trace after a trace kernel (thermal fit only): a grid of up to
-// 4 workgroups per CU, each lane taking list entries in a grid-sized stride.
-template <int NF, bool GRAD, bool DIM3, int INT, bool RESUME>
-static hipError_t launch_exact(rt_ctx *c, const Params &P, const TraceArgs &A)
-{
-    const long long grid = std::max(1LL, std::min<long long>(4LL * c->num_cu,
-                                                             ((long long)A.total + 63) / 64));
-    hipLaunchKernelGGL((exact_kernel<NF, GRAD, DIM3, INT, RESUME>), dim3((unsigned)grid),
-                       dim3(64), 0, c->stream, P, A);
-    return hipGetLastError();
-}
-
+// The FP64 re-trace after a multi-frequency trace kernel (thermal fit only;
+// with one frequency the trace kernel traces its rays again itself): n_freq
+// rows of up to 4 workgroups per CU, each lane taking list entries in a
+// row-sized stride.
 template <bool GRAD, bool DIM3, int INT, bool RESUME>
 static hipError_t dispatch_exact(rt_ctx *c, const Params &P, const TraceArgs &A)
 {
-    // one frequency: traced again by the trace kernel itself
     if (P.emission != kThSynch || P.nf == 1)
         return hipSuccess;
-    if (P.nf <= 4)
-        return launch_exact<4, GRAD, DIM3, INT, RESUME>(c, P, A);
-    if (P.nf <= 8)
-        return launch_exact<8, GRAD, DIM3, INT, RESUME>(c, P, A);
-    return launch_exact<16, GRAD, DIM3, INT, RESUME>(c, P, A);
+    const long long gx = std::max(1LL, std::min<long long>(4LL * c->num_cu,
+                                                           ((long long)A.total + 63) / 64));
+    hipLaunchKernelGGL((exact_kernel<GRAD, DIM3, INT, RESUME>), dim3((unsigned)gx, (unsigned)P.nf),
+                       dim3(64), 0, c->stream, P, A);
+    return hipGetLastError();
 }
 
 // Work items chunk by chunk (at most kMaxChunk each): with pct <= 0 all in
@@ -1152,13 +1151,13 @@ static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *fr
     A.counters = d_counters;
     A.st = RayState{};
     if (noalloc) {
-        A.xlist = (int *)(scr + kScratchXlist);
+        A.xlist = (int2 *)(scr + kScratchXlist);
     } else {
         if (!c->d_xlist) {
             if (c->capturing)
                 return fail(c, RT_E_STATE, "the FP64 re-trace's ray list is allocated by an "
                                            "eager render; render once before capturing");
-            HIPCHK(c, hipMalloc((void **)&c->d_xlist, (size_t)kMaxChunk * sizeof(int)));
+            HIPCHK(c, hipMalloc((void **)&c->d_xlist, (size_t)kMaxChunk * sizeof(int2)));
         }
         A.xlist = c->d_xlist;
     }

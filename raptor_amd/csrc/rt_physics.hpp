@@ -1025,11 +1025,13 @@ struct Accum {
     float tau[NF], ctau[NF];
     float I[NF];
     float wrel;   // sum over samples in the gate window of dI / I (see kGateMargin)
+    unsigned gmask;   // bit f: frequency f had a sample in the gate window
     double dtau[GRAD ? NF : 1][2];
     double dI[GRAD ? NF : 1][2];
     __host__ __device__ void clear()
     {
         wrel = 0.f;
+        gmask = 0u;
         for (int f = 0; f < NF; f++) {
             tau[f] = ctau[f] = 0.f;
             I[f] = 0.f;
@@ -1089,10 +1091,23 @@ __host__ __device__ inline void kahan_add(float &s, float &c, float x)
 //   dtau = nu_p alpha_nu dl / nu = g alpha_nu dl
 //   dI nu^3 = (j_nu / nu_p^2) exp(-tau) (dl / nu) nu^3 = j_nu exp(-tau) dl / g^2
 // wf (the second trace of exact_retrace): each frequency's share of the
-// samples in the gate window, instead of their sum in acc.wrel.
+// samples in the gate window, instead of their sum in acc.wrel.  The
+// accumulators' frequency f is the render's frequency f0 + f.
+// The thermal fit's emissivity factor of frequency f0 + f at a sample (the
+// part of transfer_sample that the gate-window pass below evaluates again).
+__host__ __device__ inline EmisF thermal_nu(const Params &P, const EmisPre &pre, float gdop,
+                                            float lgs, float rg3, int f)
+{
+    float nu_p = gdop * P.fnu[f];
+    if (nu_p <= 0.f)
+        nu_p = 1.f;
+    return emission_nu(pre, nu_p, fexp2(fmaf(lgs, 1.f / 6.f, P.flnu6[f])), rg3 * P.fkap[f]);
+}
+
 template <int NF, bool GRAD, int EMIS = kThSynch>
 __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F, double dlabs,
-                                                Accum<NF, GRAD> &acc, float *wf = nullptr)
+                                                Accum<NF, GRAD> &acc, float *wf = nullptr,
+                                                int f0 = 0)
 {
     const float lnth = 6.907755278982137f;   // log(1000.)
     const float dl = (float)(dlabs * P.Rg);
@@ -1109,19 +1124,24 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
     // give identical bits
     const float lgs = flog2(F.gdop * pre.inus * 0x1p37f);
     const float rg3 = rg * rg * rg;
+    // the smallest |tau - ln 1000| over the frequencies after this sample:
+    // the gate window is tested once per sample, not once per frequency (a
+    // branch per frequency held an exec mask each, and at 8 frequencies the
+    // trace kernel spilled scalar registers into vector lanes and read them
+    // back every frequency)
+    float dev = INFINITY;
 #pragma unroll
     for (int f = 0; f < NF; f++) {
-        float g = F.gdop;
-        float nu_p = g * P.fnu[f];
-        if (nu_p <= 0.f) {
-            nu_p = 1.f;
-            if constexpr (EMIS != kThSynch)
-                g = frcp(P.fnu[f]);
-        }
         if constexpr (EMIS != kThSynch) {
             static_assert(!GRAD, "the tangents are those of the thermal fit");
             if (f >= P.nf)
                 break;
+            float g = F.gdop;
+            float nu_p = g * P.fnu[f0 + f];
+            if (nu_p <= 0.f) {
+                nu_p = 1.f;
+                g = frcp(P.fnu[f0 + f]);
+            }
             const EmisX e = EMIS == kKappa
                                 ? emission_kappa(P.kc, F.B, F.mu, F.thetae, nu_p, F.ne)
                                 : emission_thsynchav(F.B, F.thetae, nu_p, F.ne);
@@ -1134,8 +1154,8 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
                 acc.I[f] += dI;
             continue;
         }
-        const EmisF e =
-            emission_nu(pre, nu_p, fexp2(fmaf(lgs, 1.f / 6.f, P.flnu6[f])), rg3 * P.fkap[f]);
+        const float g = F.gdop;
+        const EmisF e = thermal_nu(P, pre, g, lgs, rg3, f0 + f);
         const float jx = e.jh * fexp(-e.cb);
         float dtau = e.jh != 0.f ? g * dl * jx * e.kap : (e.poison ? NAN : 0.f);
         kahan_add(acc.tau[f], acc.ctau[f], dtau);
@@ -1143,6 +1163,7 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
         const float dI = (float)kE2C * jx * fexp(-tau) * dlg2;
         float dlnj0 = 0.f, dlnj1 = 0.f, dlna1 = 0.f;
         if (GRAD && e.jh != 0.f) {
+            float nu_p = g * P.fnu[f0 + f];
             float Dx = dlnj_dlnX(e.s6);
             float dlnB = 0.5f * (F.B - 1e-20f) * frcp(F.B);
             float dlnT = F.dthetae * frcp(F.thetae);
@@ -1163,15 +1184,33 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
                 acc.dI[f][1] += (double)dI * (dlnj1 - acc.dtau[f][1]);
             }
         }
-        if (fabsf(tau - lnth) < kGateMargin && dI > 0.f) {
-            // I with this sample added either way; 0 (a first sample at the
-            // gate) gives an infinite share, which flags the ray
-            const float Iw = tau < lnth ? acc.I[f] : acc.I[f] + dI;
-            const float share = dI * frcp(Iw);
-            if (wf)
-                wf[f] += share;
-            else
-                acc.wrel += share;
+        dev = fminf(dev, fabsf(tau - lnth));
+    }
+    if constexpr (EMIS == kThSynch) {
+        if (__builtin_expect(dev < kGateMargin, 0)) {
+            // a frequency's tau is in the gate window: its dI again (the same
+            // arithmetic), and its share of the intensity
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                const float tau = acc.tau[f];
+                if (!(fabsf(tau - lnth) < kGateMargin))
+                    continue;
+                const EmisF e = thermal_nu(P, pre, F.gdop, lgs, rg3, f0 + f);
+                const float jx = e.jh * fexp(-e.cb);
+                const float dI = (float)kE2C * jx * fexp(-tau) * dlg2;
+                if (!(dI > 0.f))
+                    continue;
+                // I with this sample added either way; 0 (a first sample at
+                // the gate) gives an infinite share, which flags the ray
+                const float Iw = tau < lnth ? acc.I[f] : acc.I[f] + dI;
+                const float share = dI * frcp(Iw);
+                if (wf) {
+                    wf[f] += share;
+                } else {
+                    acc.wrel += share;
+                    acc.gmask |= 1u << f;
+                }
+            }
         }
     }
 }
@@ -1250,12 +1289,14 @@ __host__ __device__ inline bool fluid_sample_f64(const Params &P, const float4 *
 // relative, which this path reproduces, since it exists to reach the gate
 // decisions of the reference's FP64 optical depth.  tau lives in the
 // accumulator's (tau, ctau) pair; I and, with GRAD, the tangents accumulate
-// as in transfer_sample.  Returns false where the sigma cut skips the sample.
+// as in transfer_sample (accumulator f: the render's frequency f0 + f).
+// Returns false where the sigma cut skips the sample.
 template <int NF, bool GRAD, bool DIM3>
 __host__ __device__ inline bool transfer_sample_f64(const Params &P,
                                                     const float4 *__restrict__ prims,
                                                     const Point &pt, const double y[8],
-                                                    double dlabs, Accum<NF, GRAD> &acc)
+                                                    double dlabs, Accum<NF, GRAD> &acc,
+                                                    int f0 = 0)
 {
     Fluid64 F;
     if (!fluid_sample_f64<DIM3>(P, prims, pt, y, F))
@@ -1272,9 +1313,9 @@ __host__ __device__ inline bool transfer_sample_f64(const Params &P,
     for (int f = 0; f < NF; f++) {
         // unrolled (the accumulators stay in registers); frequencies past
         // P.nf are skipped
-        if (f >= P.nf)
+        if (f0 + f >= P.nf)
             continue;
-        const double nu = P.freq[f];
+        const double nu = P.freq[f0 + f];
         double nu_p = F.gdop * nu;
         if (nu_p <= 0.)
             nu_p = 1.;
@@ -1334,14 +1375,22 @@ __host__ __device__ inline bool transfer_sample_f64(const Params &P,
 // at the gate.
 constexpr float kOpaqueTau = 6.907755278982137f + 2.f * kGateMargin;
 
+// The smallest optical depth over the accumulators against the gate: one
+// comparison instead of one per frequency (their AND chain held a mask
+// register pair per frequency, which at 8 frequencies pushed the trace
+// kernel into spilling scalar registers).  Accumulators past the render's
+// frequencies repeat its last frequency (fill_params), so they change
+// nothing.  fminf ignores a NaN operand, so a frequency whose tau is NaN
+// (after which no sample adds, like past the gate) does not keep the ray
+// going, and all-NaN gives NaN, which ends it.
 template <int NF, bool GRAD>
 __host__ __device__ inline bool opaque(const Params &P, const Accum<NF, GRAD> &acc)
 {
-    bool o = true;
+    float m = acc.tau[0];
 #pragma unroll
-    for (int f = 0; f < NF; f++)
-        o = o && (f >= P.nf || !(acc.tau[f] < kOpaqueTau));
-    return o;
+    for (int f = 1; f < NF; f++)
+        m = fminf(m, acc.tau[f]);
+    return !(m < kOpaqueTau);
 }
 
 template <int NF, bool GRAD>
@@ -1482,11 +1531,12 @@ __host__ __device__ inline bool ray_step(const Params &P, const float4 *__restri
 // launch of that frequency alone makes -- takes x's outputs (FP64 tau and
 // tangents, intensity summed in FP32), the others keep the FP32 ones (R.acc,
 // the same as the first trace's), so that batched and single-frequency
-// launches give the same bits.
+// launches give the same bits.  The accumulators' frequency f is the
+// render's frequency f0 + f (exact_kernel traces one frequency per lane).
 template <int NF, bool GRAD, bool DIM3, int INT>
 __host__ __device__ inline void exact_retrace(const Params &P, const float4 *__restrict__ prims,
                                               Ray<NF, GRAD> &R, Accum<NF, GRAD> &x,
-                                              float (&wf)[NF])
+                                              float (&wf)[NF], int f0 = 0)
 {
     x.clear();
 #pragma unroll
@@ -1500,9 +1550,9 @@ __host__ __device__ inline void exact_retrace(const Params &P, const float4 *__r
             Fluid F;
             if (fluid_sample<DIM3>(P, prims, R.pt, R.y, F)) {
                 R.samples++;
-                transfer_sample<NF, GRAD, kThSynch>(P, F, fabs(dl), R.acc, wf);
+                transfer_sample<NF, GRAD, kThSynch>(P, F, fabs(dl), R.acc, wf, f0);
             }
-            transfer_sample_f64<NF, GRAD, DIM3>(P, prims, R.pt, R.y, fabs(dl), x);
+            transfer_sample_f64<NF, GRAD, DIM3>(P, prims, R.pt, R.y, fabs(dl), x, f0);
             R.terminate = R.terminate || (opaque(P, R.acc) && opaque_f64(P, x));
         }
         R.steps++;
@@ -1595,7 +1645,13 @@ __host__ __device__ inline bool ray_step_rk2_overlap(const Params &P,
 // ray_step and ray_step_rk2_overlap with the transfer chosen per lane: the
 // FP64 one (transfer_sample_f64) where `exact`, the FP32 one elsewhere; the
 // geodesic step is shared, so a wave with a lane in its second trace
-// (needs_exact) runs the FP64 sample beside the others' FP32 sample.
+// (needs_exact) runs the FP64 sample beside the others' FP32 sample.  The
+// FP64 branch is marked unlikely (about 1 ray in 3000 takes it) and takes
+// the next step's first acceleration after its sample, so that neither the
+// FP32 fluid state nor that acceleration is live across the FP64 sample's
+// registers: the register allocator then puts its spills in that branch
+// (the 3D image kernel at 1024^2: 8.5 -> 7.7 ms).  Same arithmetic as
+// before, the same bits.
 template <int NF, bool GRAD, bool DIM3, int INT>
 __host__ __device__ inline bool ray_step_mixed(const Params &P, const float4 *__restrict__ prims,
                                                Ray<NF, GRAD> &R, bool exact)
@@ -1604,7 +1660,7 @@ __host__ __device__ inline bool ray_step_mixed(const Params &P, const float4 *__
     if (!ray_advance<INT>(P, R, dl))
         return false;
     if (in_grid(P, R.y)) {
-        if (exact) {
+        if (__builtin_expect(exact, 0)) {
             if (transfer_sample_f64<NF, GRAD, DIM3>(P, prims, R.pt, R.y, fabs(dl), R.acc)) {
                 R.samples++;
                 R.terminate = R.terminate || opaque_f64(P, R.acc);
@@ -1632,18 +1688,21 @@ __host__ __device__ inline bool ray_step_rk2_overlap_mixed(const Params &P,
     if (!ray_advance_from(P, R, dl, A0))
         return false;
     const bool inside = in_grid(P, R.y);
-    Fluid F;
-    const bool sampled = fluid_sample<DIM3>(P, prims, R.pt, R.y, F);
-    geodesic_accel(R.pt, P.a, P.hslope, R.y + 4, A0);
-    if (exact) {
+    if (__builtin_expect(exact, 0)) {
         if (inside && transfer_sample_f64<NF, GRAD, DIM3>(P, prims, R.pt, R.y, fabs(dl), R.acc)) {
             R.samples++;
             R.terminate = R.terminate || opaque_f64(P, R.acc);
         }
-    } else if (inside && sampled) {
-        R.samples++;
-        transfer_sample<NF, GRAD, kThSynch>(P, F, fabs(dl), R.acc);
-        R.terminate = R.terminate || opaque(P, R.acc);
+        geodesic_accel(R.pt, P.a, P.hslope, R.y + 4, A0);
+    } else {
+        Fluid F;
+        const bool sampled = fluid_sample<DIM3>(P, prims, R.pt, R.y, F);
+        geodesic_accel(R.pt, P.a, P.hslope, R.y + 4, A0);
+        if (inside && sampled) {
+            R.samples++;
+            transfer_sample<NF, GRAD, kThSynch>(P, F, fabs(dl), R.acc);
+            R.terminate = R.terminate || opaque(P, R.acc);
+        }
     }
     R.steps++;
     return true;

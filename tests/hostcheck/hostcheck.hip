@@ -46,24 +46,54 @@ static void trace_to_end(const Params &P, const float4 *prims, Ray<NF, GRAD> &R,
             // trace's with several frequencies (exact_kernel writes the image
             // only) and are the second's with one (the trace kernel's lane
             // traces it again itself)
-            const int steps1 = R.steps, samples1 = R.samples;
-            restart(R);
-            Accum<NF, GRAD> x;
-            float wf[NF];
-            if (P.n3 > 1)
-                exact_retrace<NF, GRAD, true, INT>(P, prims, R, x, wf);
-            else
-                exact_retrace<NF, GRAD, false, INT>(P, prims, R, x, wf);
-            if (P.nf > 1) {
-                R.steps = steps1;
-                R.samples = samples1;
-            }
-            for (int f = 0; f < P.nf; f++) {
-                if (wf[f] > kGateFlip) {
-                    R.acc.I[f] = x.I[f];
+            if (P.nf == 1) {
+                restart(R);
+                Accum<NF, GRAD> x;
+                float wf[NF];
+                if (P.n3 > 1)
+                    exact_retrace<NF, GRAD, true, INT>(P, prims, R, x, wf);
+                else
+                    exact_retrace<NF, GRAD, false, INT>(P, prims, R, x, wf);
+                if (wf[0] > kGateFlip) {
+                    R.acc.I[0] = x.I[0];
                     if (GRAD) {
-                        R.acc.dI[f][0] = x.dI[f][0];
-                        R.acc.dI[f][1] = x.dI[f][1];
+                        R.acc.dI[0][0] = x.dI[0][0];
+                        R.acc.dI[0][1] = x.dI[0][1];
+                    }
+                }
+                return;
+            }
+            // several frequencies: one second trace per frequency that had a
+            // sample in the gate window, with that frequency alone, as
+            // exact_kernel's lanes do
+            const unsigned mask = R.acc.gmask;
+            for (int f = 0; f < P.nf; f++) {
+                if (!((mask >> f) & 1u))
+                    continue;
+                Ray<NF, GRAD> S;
+                restart(S);
+                Ray<1, GRAD> R1;
+                for (int q = 0; q < 8; q++)
+                    R1.y[q] = S.y[q];
+                R1.pt = S.pt;
+                R1.thetadot_sign = S.thetadot_sign;
+                R1.beta_sign = S.beta_sign;
+                R1.steps = S.steps;
+                R1.turns = S.turns;
+                R1.samples = 0;
+                R1.terminate = false;
+                R1.acc.clear();
+                Accum<1, GRAD> x;
+                float wf[1];
+                if (P.n3 > 1)
+                    exact_retrace<1, GRAD, true, INT>(P, prims, R1, x, wf, f);
+                else
+                    exact_retrace<1, GRAD, false, INT>(P, prims, R1, x, wf, f);
+                if (wf[0] > kGateFlip) {
+                    R.acc.I[f] = x.I[0];
+                    if (GRAD) {
+                        R.acc.dI[f][0] = x.dI[0][0];
+                        R.acc.dI[f][1] = x.dI[0][1];
                     }
                 }
             }

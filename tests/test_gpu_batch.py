@@ -349,6 +349,9 @@ def test_autotune_picks_a_valid_config_and_caches_it(tracer, tmp_path):
                                                            approach=(100,)),
                         reps=1, path=path)
     assert len(res["table"]) == 4 and res["ms"] > 0
+    # the leading shapes timed again; the result is the fastest of those
+    assert len(res["refined"]) == 4
+    assert res["ms"] == min(ms for _, ms in res["refined"])
     assert tracer.get_launch_config()["tile"] == res["config"]["tile"]
     tracer.set_launch_config()
     assert autotune.apply_cached(tracer, 1, 1, path=path) == res["config"]
