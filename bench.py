@@ -267,6 +267,7 @@ def bench_c1(args):
 
     roof = _rooflines(cnt, 1, kms, per, grid_step, kernel=trace_name) if thermal else None
     traffic = issue = None
+    prof = {}
     tfile = os.path.join(ROOT, "profiles", "trace_kernel_traffic.json")
     if roof and W == 1024 and split and os.path.exists(tfile):
         with open(tfile) as fh:
@@ -284,6 +285,9 @@ def bench_c1(args):
             "traffic_note": "HBM bytes per launch of the trace kernel from rocprofv3 "
                             "FETCH_SIZE/WRITE_SIZE, copy-calibrated "
                             "(profiles/trace_kernel_traffic.json)",
+            "traffic_algorithmic": prof.get("trace_algorithmic_bytes"),
+            "render_traffic": prof.get("render_bytes"),
+            "render_traffic_algorithmic": prof.get("render_algorithmic_bytes"),
             "kernel": t["kernel"], "kernel_ms": t["kernel_ms"],
             "kernel_note": "the dominant kernel alone, HIP events around it on the library "
                            "stream; algorithmic work = its RK2 steps (from the grid-entry "

@@ -188,7 +188,9 @@ double rt_jansky_factor(const rt_ctx *ctx);
 
 /* Launch shape of the persistent trace kernel.  block: threads per
  * workgroup (64; other values are rejected).  blocks_per_cu: resident
- * workgroups per CU (0 = the occupancy limit).  refill: a lane whose ray has
+ * workgroups per CU (0 = library default: the occupancy limit, or one wave
+ * per SIMD for launches of fewer than 2^19 rays, which end on their longest
+ * rays).  refill: a lane whose ray has
  * ended claims a new one only when at least `refill` lanes of its wave are
  * idle (1..64; 0 = library default).  tile: rays are handed out in T x T
  * pixel tiles (1 = single pixels; 0 = library default, 8).  order: the order

@@ -1093,6 +1093,25 @@ __host__ __device__ inline void kahan_add(float &s, float &c, float x)
 // wf (the second trace of exact_retrace): each frequency's share of the
 // samples in the gate window, instead of their sum in acc.wrel.  The
 // accumulators' frequency f is the render's frequency f0 + f.
+// A sample of frequency f whose tau is in the gate window: its share of the
+// intensity, dI / I with I after this sample or as if it had been added
+// (0, a first sample at the gate, gives an infinite share, which flags the
+// ray), into wf[f] (the second trace) or into the accumulator's sum and mask.
+template <int NF, bool GRAD>
+__host__ __device__ inline void gate_share(Accum<NF, GRAD> &acc, int f, float tau, float dI,
+                                           float *wf)
+{
+    const float lnth = 6.907755278982137f;
+    const float Iw = tau < lnth ? acc.I[f] : acc.I[f] + dI;
+    const float share = dI * frcp(Iw);
+    if (wf) {
+        wf[f] += share;
+    } else {
+        acc.wrel += share;
+        acc.gmask |= 1u << f;
+    }
+}
+
 // The thermal fit's emissivity factor of frequency f0 + f at a sample (the
 // part of transfer_sample that the gate-window pass below evaluates again).
 __host__ __device__ inline EmisF thermal_nu(const Params &P, const EmisPre &pre, float gdop,
@@ -1124,12 +1143,14 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
     // give identical bits
     const float lgs = flog2(F.gdop * pre.inus * 0x1p37f);
     const float rg3 = rg * rg * rg;
-    // the smallest |tau - ln 1000| over the frequencies after this sample:
-    // the gate window is tested once per sample, not once per frequency (a
-    // branch per frequency held an exec mask each, and at 8 frequencies the
-    // trace kernel spilled scalar registers into vector lanes and read them
-    // back every frequency)
-    float dev = INFINITY;
+    // Several frequencies: the smallest |tau - ln 1000| over them after this
+    // sample, so that the gate window is tested once per sample, not once
+    // per frequency (a branch per frequency held an exec mask each, and at 8
+    // frequencies the trace kernel spilled scalar registers into vector lanes
+    // and read them back every frequency).  One frequency: tested where its
+    // dI is at hand (keeping the sample's emission factors for a second pass
+    // would hold their registers through the transfer).
+    [[maybe_unused]] float dev = INFINITY;
 #pragma unroll
     for (int f = 0; f < NF; f++) {
         if constexpr (EMIS != kThSynch) {
@@ -1184,9 +1205,14 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
                 acc.dI[f][1] += (double)dI * (dlnj1 - acc.dtau[f][1]);
             }
         }
-        dev = fminf(dev, fabsf(tau - lnth));
+        if constexpr (NF == 1) {
+            if (fabsf(tau - lnth) < kGateMargin && dI > 0.f)
+                gate_share(acc, f, tau, dI, wf);
+        } else {
+            dev = fminf(dev, fabsf(tau - lnth));
+        }
     }
-    if constexpr (EMIS == kThSynch) {
+    if constexpr (EMIS == kThSynch && NF > 1) {
         if (__builtin_expect(dev < kGateMargin, 0)) {
             // a frequency's tau is in the gate window: its dI again (the same
             // arithmetic), and its share of the intensity
@@ -1198,18 +1224,8 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
                 const EmisF e = thermal_nu(P, pre, F.gdop, lgs, rg3, f0 + f);
                 const float jx = e.jh * fexp(-e.cb);
                 const float dI = (float)kE2C * jx * fexp(-tau) * dlg2;
-                if (!(dI > 0.f))
-                    continue;
-                // I with this sample added either way; 0 (a first sample at
-                // the gate) gives an infinite share, which flags the ray
-                const float Iw = tau < lnth ? acc.I[f] : acc.I[f] + dI;
-                const float share = dI * frcp(Iw);
-                if (wf) {
-                    wf[f] += share;
-                } else {
-                    acc.wrel += share;
-                    acc.gmask |= 1u << f;
-                }
+                if (dI > 0.f)
+                    gate_share(acc, f, tau, dI, wf);
             }
         }
     }
