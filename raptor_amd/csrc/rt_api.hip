@@ -70,6 +70,7 @@ struct TraceArgs {
     int refill;
     int tile, ntx;      // work-item order: T x T pixel tiles, ntx tiles per row
     int per_cam;        // work items per camera
+    int unit;           // cameras are interleaved in units of this many work items
     const int *order;   // work item -> pixel within a camera, or NULL (scan)
     double *image;
     double *dimage;
@@ -104,14 +105,33 @@ __device__ inline void write_ray(const Params &P, const TraceArgs &A, int ray,
     }
 }
 
-// Work item -> (camera, pixel).  With an order table (per_cam = W*H) the
-// table gives the pixel.  Otherwise tile <= 1: row-major pixels; or T x T
-// tiles in row-major tile order, pixels row-major inside a tile, and items
-// that fall outside a ragged last tile row/column are skipped.
+// Work item -> (camera, pixel).  The cameras' work items are interleaved in
+// units of A.unit (one claim tile): unit u of the launch is unit u / n_cam of
+// camera u % n_cam, so every camera's first tiles -- the long rays near the
+// centre -- are claimed at the start of the launch (camera after camera, the
+// last camera's long rays started when most of the launch was done, and
+// ended it late); a camera's trailing partial unit comes after all whole
+// units.  Within a camera, r indexes its work items: with an order table
+// (per_cam = W*H) the table gives the pixel.  Otherwise tile <= 1:
+// row-major pixels; or T x T tiles in row-major tile order, pixels
+// row-major inside a tile, and items that fall outside a ragged last tile
+// row/column are skipped.
 __device__ inline bool work_to_ray(const Params &P, const TraceArgs &A, int w, int &ray)
 {
-    const int cam = w / A.per_cam;
-    const int r = w - cam * A.per_cam;
+    const int U = A.unit, nc = P.ncam;
+    const int full = A.per_cam / U;
+    const int head = full * U * nc;
+    int cam, r;
+    if (w < head) {
+        const int u = w / U;
+        cam = u % nc;
+        r = (u / nc) * U + (w - u * U);
+    } else {
+        const int rem = A.per_cam - full * U;   // > 0 here
+        const int x = w - head;
+        cam = x / rem;
+        r = full * U + (x - cam * rem);
+    }
     int pix;
     if (A.order) {
         pix = A.order[r];
@@ -1150,6 +1170,7 @@ static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *fr
         A.ntx = 0;
         A.per_cam = P.npix;
     }
+    A.unit = A.tile > 1 ? A.tile * A.tile : 64;
     const int total = ncam * A.per_cam;
     if (noalloc && total > kNoAllocRays)
         return fail(c, RT_E_INVALID, "rt_render_device_noalloc: %d work items (n_cam x W x H, "
