@@ -117,7 +117,8 @@ int rt_set_params(rt_ctx *ctx, const rt_params *p);
  * (raptor_harm_model.h:67-74) to HBM once.  h_prims is float32 in the
  * reference's index order p[v][i][j][k], i.e. h_prims[((v*n1+i)*n2+j)*n3+k].
  * Requires n1, n2 >= 2, n3 >= 1, R0 == 0 (the reference's inverse metric
- * assumes it, metric.c:207-231). */
+ * assumes it, metric.c:207-231), and at most 2^27 cells (4 GiB of FP32
+ * primitives; the samplers address them with 32-bit offsets). */
 int rt_load_prims(rt_ctx *ctx, const rt_grid *grid, const float *h_prims);
 
 /* ---- the pixel loop ----------------------------------------------------- */

@@ -1409,6 +1409,9 @@ int rt_load_prims(rt_ctx *c, const rt_grid *g, const float *h_prims)
     if (!(std::fabs(g->a) < 1.) || !(g->dx1 > 0.) || !(g->dx2 > 0.) ||
         (g->n3 > 1 && !(g->dx3 > 0.)))
         return fail(c, RT_E_INVALID, "need |a| < 1 and positive cell sizes");
+    if ((long long)g->n1 * g->n2 * g->n3 > kMaxCells)
+        return fail(c, RT_E_INVALID, "grids of more than 2^27 cells are not supported (%d x %d "
+                    "x %d)", g->n1, g->n2, g->n3);
     int rc = select_device(c);
     if (rc)
         return rc;

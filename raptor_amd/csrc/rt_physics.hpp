@@ -89,9 +89,15 @@ struct KappaConst {
     int tn;
 };
 
+// Largest grid the samplers address: cell offsets into the staged
+// primitives (two float4, 32 bytes, per cell) are unsigned 32-bit byte
+// offsets, 2^27 cells (4 GiB of FP32 primitives, e.g. 512^3).
+constexpr long long kMaxCells = 1LL << 27;
+
 struct Params {
     // grid (raptor_harm_model.c:120-135)
     int n1, n2, n3;
+    unsigned s1, s2;             // float4 elements per step in i (2 n2 n3) and in j (2 n3)
     double startx1, startx2, startx3;
     double dx1, dx2, dx3;
     double idx1, idx2, idx3;     // 1 / dx
@@ -618,13 +624,19 @@ __host__ __device__ inline void sample_prims(const Params &P, const float4 *__re
     const T c0 = (T(1) - f1) * (T(1) - f2), c1 = (T(1) - f1) * f2;
     const T c2 = f1 * (T(1) - f2), c3 = f1 * f2;
 #define RT_BI(f00, f01, f10, f11) ((f00) * c0 + (f01) * c1 + (f10) * c2 + (f11) * c3)
+    // float4 element e of the staged primitives: a 32-bit byte offset from
+    // the (wave-uniform) base, so that loads take the base from scalar
+    // registers and the offset from one vector register, and the cell
+    // arithmetic stays 32-bit (kMaxCells bounds it)
+    const char *const pb = (const char *)prims;
+    auto ld = [pb](unsigned e) { return *(const float4 *)(pb + (unsigned)(e * 16u)); };
+    const unsigned b00 = (unsigned)i * P.s1 + (unsigned)j * P.s2;
     if (!DIM3) {
-        const size_t b00 = (size_t)(i * P.n2 + j) * 2;
-        const size_t b10 = b00 + (size_t)P.n2 * 2;
-        float4 p00a = prims[b00], p00b = prims[b00 + 1];
-        float4 p01a = prims[b00 + 2], p01b = prims[b00 + 3];
-        float4 p10a = prims[b10], p10b = prims[b10 + 1];
-        float4 p11a = prims[b10 + 2], p11b = prims[b10 + 3];
+        const unsigned b10 = b00 + P.s1;
+        float4 p00a = ld(b00), p00b = ld(b00 + 1);
+        float4 p01a = ld(b00 + 2), p01b = ld(b00 + 3);
+        float4 p10a = ld(b10), p10b = ld(b10 + 1);
+        float4 p11a = ld(b10 + 2), p11b = ld(b10 + 3);
         v[0] = RT_BI(p00a.x, p01a.x, p10a.x, p11a.x);
         v[1] = RT_BI(p00a.y, p01a.y, p10a.y, p11a.y);
         v[2] = RT_BI(p00a.z, p01a.z, p10a.z, p11a.z);
@@ -653,18 +665,16 @@ __host__ __device__ inline void sample_prims(const Params &P, const float4 *__re
         int k = (int)fk;
         k = k < -1 ? -1 : (k > P.n3 - 1 ? P.n3 - 1 : k);
         const int k0 = k < 0 ? P.n3 - 1 : k, k1 = k + 1 == P.n3 ? 0 : k + 1;
-        const size_t n3 = P.n3, n2 = P.n2;
-        size_t b00 = ((size_t)i * n2 + j) * n3, b01 = b00 + n3;
-        size_t b10 = b00 + n2 * n3, b11 = b10 + n3;
+        const unsigned b01 = b00 + P.s2, b10 = b00 + P.s1, b11 = b10 + P.s2;
         T w[2][8];
         const int ks[2] = {k0, k1};
 #pragma unroll
         for (int q = 0; q < 2; q++) {
-            const size_t kk = ks[q];
-            float4 p00a = prims[(b00 + kk) * 2], p00b = prims[(b00 + kk) * 2 + 1];
-            float4 p01a = prims[(b01 + kk) * 2], p01b = prims[(b01 + kk) * 2 + 1];
-            float4 p10a = prims[(b10 + kk) * 2], p10b = prims[(b10 + kk) * 2 + 1];
-            float4 p11a = prims[(b11 + kk) * 2], p11b = prims[(b11 + kk) * 2 + 1];
+            const unsigned kk = 2u * (unsigned)ks[q];
+            float4 p00a = ld(b00 + kk), p00b = ld(b00 + kk + 1);
+            float4 p01a = ld(b01 + kk), p01b = ld(b01 + kk + 1);
+            float4 p10a = ld(b10 + kk), p10b = ld(b10 + kk + 1);
+            float4 p11a = ld(b11 + kk), p11b = ld(b11 + kk + 1);
             w[q][0] = RT_BI(p00a.x, p01a.x, p10a.x, p11a.x);
             w[q][1] = RT_BI(p00a.y, p01a.y, p10a.y, p11a.y);
             w[q][2] = RT_BI(p00a.z, p01a.z, p10a.z, p11a.z);
@@ -1856,9 +1866,13 @@ inline const char *fill_params(const RtParams &p, const RtGrid &g, const double 
                                const double *freqs, int nf, Params &P)
 {
     memset(&P, 0, sizeof P);
+    if ((long long)g.n1 * g.n2 * g.n3 > kMaxCells)
+        return "grids of more than 2^27 cells are not supported";
     P.n1 = g.n1;
     P.n2 = g.n2;
     P.n3 = g.n3;
+    P.s1 = (unsigned)(2LL * g.n2 * g.n3);
+    P.s2 = (unsigned)(2 * g.n3);
     P.startx1 = g.startx1;
     P.startx2 = g.startx2;
     P.startx3 = g.startx3;

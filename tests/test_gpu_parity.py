@@ -4,6 +4,8 @@ Bar (north star): total flux within 1e-4 relative, every pixel within 1e-3
 relative with an absolute floor of 1e-6 * max(I); building blocks to
 near-FP64 rounding.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -573,6 +575,14 @@ def test_error_codes(dump128):
         tr.set_params(width=4, height=4, M_unit=M_UNIT, R_high=R_HIGH)
         with pytest.raises(rt.RtError, match="rt_load_prims"):
             tr.render([60.0], [FREQ])
+        # a grid past the samplers' 32-bit cell offsets is refused before its
+        # primitives are read (the buffer passed is one cell)
+        big = rt.RtGrid(n1=1024, n2=512, n3=512, startx1=0.3, startx2=0., startx3=0.,
+                        dx1=0.01, dx2=1. / 512, dx3=2 * np.pi / 512, a=0.9375, hslope=0.3, R0=0.)
+        one = np.zeros(8, np.float32)
+        rc = tr.lib.rt_load_prims(tr.ctx, ctypes.byref(big),
+                                  one.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        assert rc != rt.RT_OK and b"2^27" in tr.lib.rt_last_error(tr.ctx)
         tr.load_dump(dump128)
         with pytest.raises(rt.RtError, match="n_freq"):
             tr.render([60.0], np.full(17, FREQ))
