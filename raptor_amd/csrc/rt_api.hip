@@ -405,7 +405,9 @@ __global__ __launch_bounds__(64, 3) void trace_kernel_3w(const Params P, const T
 // frequency's own share of the window exceeds kGateFlip it gets the FP64
 // transfer's outputs, written over its FP32 ones -- the same decision and
 // the same arithmetic as a launch of that frequency alone, whose trace
-// kernel traces the ray again itself.  A lane's second trace ends where its
+// kernel traces the ray again itself.  With one frequency in the mask its
+// share was the ray's whole sum, so the decision is known and only the FP64
+// transfer runs.  A lane's second trace ends where its
 // frequency's optical depth has passed the gate, so that it runs from the
 // grid's edge to the crossing, not to the end of the ray at every
 // frequency.  A kernel of its own: its FP64 arithmetic needs more registers
@@ -431,8 +433,15 @@ __global__ __launch_bounds__(64) void exact_kernel(const Params P, const TraceAr
         start_ray<RESUME>(P, A, ray, q, R);
         Accum<1, GRAD> x;
         float wf[1];
-        exact_retrace<1, GRAD, DIM3, INT>(P, A.prims, R, x, wf, f);
-        if (wf[0] > kGateFlip) {
+        // a ray listed with one frequency in the gate window was listed for
+        // that frequency's own share (the first trace summed no other), which
+        // is the decision: only the FP64 transfer is traced again
+        const bool alone = (e.y & (e.y - 1)) == 0;
+        if (alone)
+            exact_retrace<1, GRAD, DIM3, INT, false>(P, A.prims, R, x, wf, f);
+        else
+            exact_retrace<1, GRAD, DIM3, INT>(P, A.prims, R, x, wf, f);
+        if (alone || wf[0] > kGateFlip) {
             const size_t o = (size_t)ray * P.nf + f;
             A.image[o] = (double)x.I[0];
             if (GRAD) {

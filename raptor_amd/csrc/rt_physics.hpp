@@ -1559,7 +1559,10 @@ __host__ __device__ inline bool ray_step(const Params &P, const float4 *__restri
 // the same as the first trace's), so that batched and single-frequency
 // launches give the same bits.  The accumulators' frequency f is the
 // render's frequency f0 + f (exact_kernel traces one frequency per lane).
-template <int NF, bool GRAD, bool DIM3, int INT>
+// WF = false: the decision is known already (the first trace's window
+// share of this frequency alone, when it was the only one in the window),
+// so only the FP64 transfer is traced, to its own crossing.
+template <int NF, bool GRAD, bool DIM3, int INT, bool WF = true>
 __host__ __device__ inline void exact_retrace(const Params &P, const float4 *__restrict__ prims,
                                               Ray<NF, GRAD> &R, Accum<NF, GRAD> &x,
                                               float (&wf)[NF], int f0 = 0)
@@ -1573,13 +1576,15 @@ __host__ __device__ inline void exact_retrace(const Params &P, const float4 *__r
         if (!ray_advance<INT>(P, R, dl))
             break;
         if (in_grid(P, R.y)) {
-            Fluid F;
-            if (fluid_sample<DIM3>(P, prims, R.pt, R.y, F)) {
-                R.samples++;
-                transfer_sample<NF, GRAD, kThSynch>(P, F, fabs(dl), R.acc, wf, f0);
+            if constexpr (WF) {
+                Fluid F;
+                if (fluid_sample<DIM3>(P, prims, R.pt, R.y, F)) {
+                    R.samples++;
+                    transfer_sample<NF, GRAD, kThSynch>(P, F, fabs(dl), R.acc, wf, f0);
+                }
             }
             transfer_sample_f64<NF, GRAD, DIM3>(P, prims, R.pt, R.y, fabs(dl), x, f0);
-            R.terminate = R.terminate || (opaque(P, R.acc) && opaque_f64(P, x));
+            R.terminate = R.terminate || ((!WF || opaque(P, R.acc)) && opaque_f64(P, x));
         }
         R.steps++;
     }

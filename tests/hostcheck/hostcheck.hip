@@ -85,11 +85,16 @@ static void trace_to_end(const Params &P, const float4 *prims, Ray<NF, GRAD> &R,
                 R1.acc.clear();
                 Accum<1, GRAD> x;
                 float wf[1];
-                if (P.n3 > 1)
+                const bool alone = (mask & (mask - 1)) == 0;
+                if (alone && P.n3 > 1)
+                    exact_retrace<1, GRAD, true, INT, false>(P, prims, R1, x, wf, f);
+                else if (alone)
+                    exact_retrace<1, GRAD, false, INT, false>(P, prims, R1, x, wf, f);
+                else if (P.n3 > 1)
                     exact_retrace<1, GRAD, true, INT>(P, prims, R1, x, wf, f);
                 else
                     exact_retrace<1, GRAD, false, INT>(P, prims, R1, x, wf, f);
-                if (wf[0] > kGateFlip) {
+                if (alone || wf[0] > kGateFlip) {
                     R.acc.I[f] = x.I[0];
                     if (GRAD) {
                         R.acc.dI[f][0] = x.dI[0][0];
