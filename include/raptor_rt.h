@@ -176,10 +176,11 @@ double rt_last_kernel_ms(const rt_ctx *ctx);
  * context's stream (synchronises it): out[0] the approach kernel with the
  * queue reset (about 0 when no approach ran), out[1] the persistent trace
  * kernel, out[2] the FP64 re-trace of the rays at the tau gate, out[3] the
- * spectrum reduction (about 0 when none was asked for).  Kept for renders of
- * one launch chunk (at most 2^23 rays); RT_E_STATE otherwise.  No
- * counterpart in the reference, which times its pixel loop with clock()
- * (core.c:91-93,116). */
+ * spectrum reduction (about 0 when none was asked for), out[4] the ring
+ * order's lead kernel, which runs beside the others on a second stream (0
+ * when none ran).  out holds 5 doubles.  Kept for renders of one launch
+ * chunk (at most 2^23 rays); RT_E_STATE otherwise.  No counterpart in the
+ * reference, which times its pixel loop with clock() (core.c:91-93,116). */
 int rt_last_kernel_times(rt_ctx *ctx, double *out);
 
 /* JANSKY_FACTOR of utilities.c:32-42 for the current parameters. */
@@ -197,8 +198,13 @@ double rt_jansky_factor(const rt_ctx *ctx);
  * pixel tiles (1 = single pixels; 0 = library default, 8).  order: the order
  * of the tiles, RT_ORDER_CENTRE (by distance from the image centre, so the
  * long rays near the photon ring start first and the short escaping rays
- * fill the end of the launch) or RT_ORDER_SCAN (row-major); 0 = library
- * default (centre).  approach: the share of the rays, in claim order,
+ * fill the end of the launch), RT_ORDER_SCAN (row-major) or RT_ORDER_RING:
+ * the centre order, and for renders of one camera and one frequency that
+ * run the split launch below in one chunk, the pixels within 0.3 M of the
+ * camera's critical curve (the photon ring, the longest rays) are traced
+ * from the camera by a lead kernel on a second stream of the context from
+ * the start of the render, beside the approach kernel, instead of after it;
+ * 0 = library default (ring).  approach: the share of the rays, in claim order,
  * whose camera-to-grid leg runs first in a kernel of its own: 1..100
  * percent, RT_APPROACH_NONE for none, 0 = library default (all rays for
  * launches of at least 2^20 rays, none below).  Those rays are
@@ -207,11 +213,12 @@ double rt_jansky_factor(const rt_ctx *ctx);
  * persistent kernel, whose waves then do no camera-to-grid steps beside
  * rays inside the grid; it traces the other rays from the camera, and
  * these fill the time in which the longest resumed rays finish.  None of
- * these changes any output bit.  The centre order for a new image shape, and
- * the buffer of saved rays, are made by an eager render, so render once
- * before capturing a graph. */
+ * these changes any output bit.  The centre and ring orders for a new image
+ * shape (and, ring, camera), and the buffer of saved rays, are made by an
+ * eager render, so render once before capturing a graph. */
 #define RT_ORDER_CENTRE 1
 #define RT_ORDER_SCAN 2
+#define RT_ORDER_RING 3
 #define RT_APPROACH_NONE (-1)
 typedef struct {
     int block;

@@ -18,14 +18,14 @@ import os
 
 import numpy as np
 
-from .rt import APPROACH_NONE, ORDER_CENTRE, ORDER_SCAN
+from .rt import APPROACH_NONE, ORDER_RING, ORDER_SCAN
 
 CACHE = os.environ.get("RAPTOR_AUTOTUNE_CACHE",
                        os.path.join(os.path.expanduser("~"), ".cache", "raptor_amd",
                                     "autotune.json"))
 
 DEFAULT_SPACE = dict(blocks_per_cu=(0, 4, 6), refill=(1, 4, 32), tile=(1, 4, 8, 16),
-                     order=(ORDER_CENTRE, ORDER_SCAN), approach=(APPROACH_NONE, 100))
+                     order=(ORDER_RING, ORDER_SCAN), approach=(APPROACH_NONE, 100))
 
 
 def workload_key(tracer, n_cam, n_freq, grad, dim3):

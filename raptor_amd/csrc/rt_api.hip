@@ -661,6 +661,10 @@ __global__ void stage_prims(const float *soa, size_t ncell, float4 *aos)
 
 }  // namespace
 
+// claim-order table key: W, H, tile, pixel size bits (x, y), and for the
+// ring order the spin and inclination bits (0 otherwise)
+static constexpr int kOrderKey = 7;
+
 struct rt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -668,6 +672,13 @@ struct rt_ctx {
     // boundaries between a one-chunk render's kernels: after the approach,
     // after the trace kernel, after the FP64 re-trace (rt_last_kernel_times)
     hipEvent_t evA = nullptr, evT = nullptr, evX = nullptr;
+    // the ring order's lead launch: a second stream (default priority),
+    // forked from and joined back into `stream` by the two events, and the
+    // lead kernel's own time (evL0, evL1; lead_timed: they hold the last
+    // render's)
+    hipStream_t lead_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, evL0 = nullptr, evL1 = nullptr;
+    bool lead_timed = false;
     int last_chunks = 0;   // launch chunks of the last eager render
     int num_cu = 0;
     bool have_params = false, have_grid = false;
@@ -689,8 +700,9 @@ struct rt_ctx {
     // A table is never changed in place, and one that a captured graph
     // launches with is pinned until that graph is destroyed.
     struct OrderTable {
-        long long key[5];
+        long long key[kOrderKey];
         int *d;
+        int nlead;                 // ring order: its first nlead entries are the lead rays
         int refs;                  // live graphs that use the table
         unsigned long long used;   // last use, for eviction
     };
@@ -847,7 +859,8 @@ static int make_params(rt_ctx *c, const double *incl, int ncam, const double *fr
 }
 
 template <int NF, bool GRAD, bool DIM3, int INT, bool RESUME, int EMIS = kThSynch>
-static hipError_t launch_trace(rt_ctx *c, const Params &P, const TraceArgs &A)
+static hipError_t launch_trace(rt_ctx *c, const Params &P, const TraceArgs &A,
+                               hipStream_t stream)
 {
     void (*kern)(const Params, const TraceArgs) = trace_kernel<NF, GRAD, DIM3, INT, RESUME, EMIS>;
     if constexpr (EMIS == kThSynch && three_waves<NF, GRAD, DIM3, INT>()) {
@@ -873,33 +886,34 @@ static hipError_t launch_trace(rt_ctx *c, const Params &P, const TraceArgs &A)
     long long grid = (long long)per_cu * c->num_cu;
     long long need = ((long long)A.total + 63) / 64;
     grid = std::max(1LL, std::min(grid, need));
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), 0, c->stream, P, A);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), 0, stream, P, A);
     return hipGetLastError();
 }
 
 // The thermal fit is built for 1, 4, 8 and 16 frequencies, image and
 // gradient; the other emission models (image only) for 1 and 16.
 template <bool GRAD, bool DIM3, int INT, bool RESUME>
-static hipError_t dispatch_nf(rt_ctx *c, const Params &P, const TraceArgs &A)
+static hipError_t dispatch_nf(rt_ctx *c, const Params &P, const TraceArgs &A,
+                              hipStream_t st)
 {
     if (P.emission != kThSynch) {
         if constexpr (GRAD) {
             return hipErrorInvalidValue;   // refused before the launch
         } else {
             if (P.emission == kKappa)
-                return P.nf == 1 ? launch_trace<1, false, DIM3, INT, RESUME, kKappa>(c, P, A)
-                                 : launch_trace<16, false, DIM3, INT, RESUME, kKappa>(c, P, A);
-            return P.nf == 1 ? launch_trace<1, false, DIM3, INT, RESUME, kThSynchAv>(c, P, A)
-                             : launch_trace<16, false, DIM3, INT, RESUME, kThSynchAv>(c, P, A);
+                return P.nf == 1 ? launch_trace<1, false, DIM3, INT, RESUME, kKappa>(c, P, A, st)
+                                 : launch_trace<16, false, DIM3, INT, RESUME, kKappa>(c, P, A, st);
+            return P.nf == 1 ? launch_trace<1, false, DIM3, INT, RESUME, kThSynchAv>(c, P, A, st)
+                             : launch_trace<16, false, DIM3, INT, RESUME, kThSynchAv>(c, P, A, st);
         }
     }
     if (P.nf == 1)
-        return launch_trace<1, GRAD, DIM3, INT, RESUME>(c, P, A);
+        return launch_trace<1, GRAD, DIM3, INT, RESUME>(c, P, A, st);
     if (P.nf <= 4)
-        return launch_trace<4, GRAD, DIM3, INT, RESUME>(c, P, A);
+        return launch_trace<4, GRAD, DIM3, INT, RESUME>(c, P, A, st);
     if (P.nf <= 8)
-        return launch_trace<8, GRAD, DIM3, INT, RESUME>(c, P, A);
-    return launch_trace<16, GRAD, DIM3, INT, RESUME>(c, P, A);
+        return launch_trace<8, GRAD, DIM3, INT, RESUME>(c, P, A, st);
+    return launch_trace<16, GRAD, DIM3, INT, RESUME>(c, P, A, st);
 }
 
 // The FP64 re-trace after a multi-frequency trace kernel (thermal fit only;
@@ -924,9 +938,32 @@ static hipError_t dispatch_exact(rt_ctx *c, const Params &P, const TraceArgs &A)
 // which resumes those and traces the rest from the camera (a chunk saves at
 // most A.st.n rays).  After each, the rays listed for the FP64 transfer.
 template <bool GRAD, bool DIM3, int INT>
-static hipError_t run_launches(rt_ctx *c, const Params &P, TraceArgs A, int total, int pct)
+static hipError_t run_launches(rt_ctx *c, const Params &P, TraceArgs A, int total, int pct,
+                               const TraceArgs *lead)
 {
     hipError_t e;
+    if (lead) {
+        // the ring's rays from the camera on the lead stream, from the start
+        // of the render; joined back after the chunk's kernels below (one
+        // chunk: the ring order is used for launches of at most kMaxChunk)
+        const bool timed = !c->capturing;
+        if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess ||
+            (e = hipStreamWaitEvent(c->lead_stream, c->ev_fork, 0)) != hipSuccess)
+            return e;
+        if (timed && (e = hipEventRecord(c->evL0, c->lead_stream)) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, c->lead_stream, lead->queue);
+        if ((e = hipGetLastError()) != hipSuccess ||
+            (e = dispatch_nf<GRAD, DIM3, INT, false>(c, P, *lead, c->lead_stream)) != hipSuccess)
+            return e;
+        if (timed && (e = hipEventRecord(c->evL1, c->lead_stream)) != hipSuccess)
+            return e;
+        if ((e = hipEventRecord(c->ev_join, c->lead_stream)) != hipSuccess)
+            return e;
+        c->lead_timed = timed;
+    } else if (!c->capturing) {
+        c->lead_timed = false;
+    }
     const long long chunk =
         pct <= 0 ? (long long)kMaxChunk
                  : std::min<long long>(kMaxChunk, (long long)A.st.n * 100 / pct);   // saves <= st.n
@@ -946,8 +983,8 @@ static hipError_t run_launches(rt_ctx *c, const Params &P, TraceArgs A, int tota
             return e;
         if (mark && (e = hipEventRecord(c->evA, c->stream)) != hipSuccess)
             return e;
-        e = A.nsaved > 0 ? dispatch_nf<GRAD, DIM3, INT, true>(c, P, A)
-                         : dispatch_nf<GRAD, DIM3, INT, false>(c, P, A);
+        e = A.nsaved > 0 ? dispatch_nf<GRAD, DIM3, INT, true>(c, P, A, c->stream)
+                         : dispatch_nf<GRAD, DIM3, INT, false>(c, P, A, c->stream);
         if (e != hipSuccess)
             return e;
         if (mark && (e = hipEventRecord(c->evT, c->stream)) != hipSuccess)
@@ -959,17 +996,20 @@ static hipError_t run_launches(rt_ctx *c, const Params &P, TraceArgs A, int tota
         if (mark && (e = hipEventRecord(c->evX, c->stream)) != hipSuccess)
             return e;
     }
+    if (lead && (e = hipStreamWaitEvent(c->stream, c->ev_join, 0)) != hipSuccess)
+        return e;
     return hipSuccess;
 }
 
 template <bool GRAD, bool DIM3>
-static hipError_t dispatch_int(rt_ctx *c, const Params &P, const TraceArgs &A, int total, int pct)
+static hipError_t dispatch_int(rt_ctx *c, const Params &P, const TraceArgs &A, int total, int pct,
+                               const TraceArgs *lead)
 {
     if (c->params.integrator == RT_INT_RK4)
-        return run_launches<GRAD, DIM3, kRK4>(c, P, A, total, pct);
+        return run_launches<GRAD, DIM3, kRK4>(c, P, A, total, pct, lead);
     if (c->params.integrator == RT_INT_VERLET)
-        return run_launches<GRAD, DIM3, kVerlet>(c, P, A, total, pct);
-    return run_launches<GRAD, DIM3, kRK2>(c, P, A, total, pct);
+        return run_launches<GRAD, DIM3, kVerlet>(c, P, A, total, pct, lead);
+    return run_launches<GRAD, DIM3, kRK2>(c, P, A, total, pct, lead);
 }
 
 static int ensure_buffer(rt_ctx *c, double **buf, size_t *cap, size_t bytes)
@@ -1019,35 +1059,98 @@ static int ensure_state(rt_ctx *c, long long slots, RayState *out)
 // times longer than escaping rays -- are claimed first, so the launch does
 // not end waiting on a few long rays that started late.  Built on the host
 // once per image size and tile, kept in HBM (4 bytes per pixel).
-static const int *find_order(rt_ctx *c, const long long key[5])
+//
+// Ring order (one camera, split launch): the same, with the pixels within
+// kRingWidth of the camera's critical curve (Bardeen's photon-orbit curve
+// for the spin and inclination) moved to the front, in centre-out order
+// among themselves.  Those are the longest rays (they orbit the hole; where
+// the curve crosses the projected spin axis they also pass the pole, where
+// steps shrink with min(X2, 1 - X2)): the lead kernel traces them from the
+// camera on a second stream from the start of the render, beside the
+// approach kernel, instead of after it.
+static constexpr double kRingWidth = 0.3;   // M: 12304 of 1024^2 pixels at 40 M, a = 0.9375, i = 60
+
+static const rt_ctx::OrderTable *find_order(rt_ctx *c, const long long key[kOrderKey])
 {
     for (auto &t : c->orders) {
-        if (memcmp(key, t.key, 5 * sizeof(long long)) == 0) {
+        if (memcmp(key, t.key, kOrderKey * sizeof(long long)) == 0) {
             t.used = ++c->use_clock;
             if (c->capturing)
                 pin(c, t.d);
-            return t.d;
+            return &t;
         }
     }
     return nullptr;
 }
 
-static void order_key(int W, int H, int T, double sx, double sy, long long key[5])
+static void order_key(int W, int H, int T, double sx, double sy, bool ring, double a,
+                      double incl, long long key[kOrderKey])
 {
-    long long bx, by;
-    memcpy(&bx, &sx, sizeof bx);
-    memcpy(&by, &sy, sizeof by);
+    const double v[4] = {sx, sy, ring ? a : 0., ring ? incl : 0.};
     key[0] = W;
     key[1] = H;
-    key[2] = T;
-    key[3] = bx;
-    key[4] = by;
+    key[2] = T | (ring ? 1 << 30 : 0);
+    memcpy(key + 3, v, sizeof v);
 }
 
-static int ensure_order(rt_ctx *c, int W, int H, int T, double sx, double sy, const int **out)
+// Pixels (row-major) within `width` (M) of the critical curve seen by a camera
+// at inclination incl (degrees) around spin a: the impact parameters
+// (alpha, beta) of the spherical photon orbits, alpha = -xi / sin i,
+// beta^2 = eta + a^2 cos^2 i - xi^2 cot^2 i with xi(r), eta(r) of Bardeen
+// (1973) for r between the prograde and retrograde circular photon orbits
+// (a circle of radius sqrt(27) for a = 0), sampled densely and each sample
+// marking the pixel centres within `width` of it.  Pixel centres as
+// pixel_impact: alpha = -cam_x/2 + (x + 1/2) sx, beta = -cam_y/2 + (y + 1/2) sy.
+static void ring_mask(int W, int H, double sx, double sy, double a, double incl,
+                      double width, std::vector<char> &mask)
 {
-    long long key[5];
-    order_key(W, H, T, sx, sy, key);
+    mask.assign((size_t)W * H, 0);
+    const double si = std::sin(incl * kPi / 180.), ci = std::cos(incl * kPi / 180.);
+    auto mark = [&](double al, double be) {
+        const int x0 = std::max(0, (int)std::floor((al - width + 0.5 * W * sx) / sx - 0.5));
+        const int x1 = std::min(W - 1, (int)std::ceil((al + width + 0.5 * W * sx) / sx - 0.5));
+        const int y0 = std::max(0, (int)std::floor((be - width + 0.5 * H * sy) / sy - 0.5));
+        const int y1 = std::min(H - 1, (int)std::ceil((be + width + 0.5 * H * sy) / sy - 0.5));
+        for (int y = y0; y <= y1; y++) {
+            const double dy = -0.5 * H * sy + (y + 0.5) * sy - be;
+            for (int x = x0; x <= x1; x++) {
+                const double dx = -0.5 * W * sx + (x + 0.5) * sx - al;
+                if (dx * dx + dy * dy < width * width)
+                    mask[(size_t)y * W + x] = 1;
+            }
+        }
+    };
+    const int n = 1 << 14;
+    if (std::fabs(a) < 1e-4) {
+        for (int q = 0; q < n; q++) {
+            const double ph = 2. * kPi * q / n;
+            mark(std::sqrt(27.) * std::cos(ph), std::sqrt(27.) * std::sin(ph));
+        }
+        return;
+    }
+    const double r1 = 2. * (1. + std::cos(2. / 3. * std::acos(-a)));
+    const double r2 = 2. * (1. + std::cos(2. / 3. * std::acos(a)));
+    const double lo = std::min(r1, r2), hi = std::max(r1, r2);
+    for (int q = 0; q <= n; q++) {
+        const double r = lo + (hi - lo) * q / n;
+        const double xi = (r * r * (3. - r) - a * a * (r + 1.)) / (a * (r - 1.));
+        const double eta = r * r * r * (4. * a * a - r * (r - 3.) * (r - 3.)) /
+                           (a * a * (r - 1.) * (r - 1.));
+        const double b2 = eta + a * a * ci * ci - xi * xi * (ci * ci) / (si * si);
+        if (!(b2 >= 0.))
+            continue;
+        mark(-xi / si, std::sqrt(b2));
+        mark(-xi / si, -std::sqrt(b2));
+    }
+}
+
+// ring: incl and a give the camera; the table's first t->nlead entries are
+// then the ring's pixels
+static int ensure_order(rt_ctx *c, int W, int H, int T, double sx, double sy, bool ring,
+                        double a, double incl, const rt_ctx::OrderTable **out)
+{
+    long long key[kOrderKey];
+    order_key(W, H, T, sx, sy, ring, a, incl, key);
     if ((*out = find_order(c, key)))
         return RT_OK;
     if (c->capturing)
@@ -1074,10 +1177,13 @@ static int ensure_order(rt_ctx *c, int W, int H, int T, double sx, double sy, co
     const size_t n = (size_t)W * H;
     std::vector<Item> items;
     std::vector<int> order;
+    std::vector<char> lead;
     try {
         items.resize(n);
         order.resize(n);
         c->orders.reserve(c->orders.size() + 1);
+        if (ring)
+            ring_mask(W, H, sx, sy, a, incl, kRingWidth, lead);
     } catch (const std::bad_alloc &) {
         return fail(c, RT_E_NOMEM, "out of host memory for the claim order");
     }
@@ -1101,11 +1207,23 @@ static int ensure_order(rt_ctx *c, int W, int H, int T, double sx, double sy, co
             return a.tile < b.tile;
         return a.pix < b.pix;
     });
-    for (size_t i = 0; i < n; i++)
-        order[i] = items[i].pix;
+    size_t nlead = 0;
+    if (ring) {
+        for (size_t i = 0; i < n; i++)
+            if (lead[items[i].pix])
+                order[nlead++] = items[i].pix;
+        size_t k = nlead;
+        for (size_t i = 0; i < n; i++)
+            if (!lead[items[i].pix])
+                order[k++] = items[i].pix;
+    } else {
+        for (size_t i = 0; i < n; i++)
+            order[i] = items[i].pix;
+    }
     rt_ctx::OrderTable t;
     memcpy(t.key, key, sizeof key);
     t.d = nullptr;
+    t.nlead = (int)nlead;
     t.refs = 0;
     t.used = ++c->use_clock;
     HIPCHK(c, hipMalloc((void **)&t.d, n * sizeof(int)));
@@ -1119,7 +1237,7 @@ static int ensure_order(rt_ctx *c, int W, int H, int T, double sx, double sy, co
         return fail(c, RT_E_HIP, "claim order upload: %s", hipGetErrorString(e));
     }
     c->orders.push_back(t);
-    *out = t.d;
+    *out = &c->orders.back();
     return RT_OK;
 }
 
@@ -1153,16 +1271,33 @@ static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *fr
     A.refill = std::min(64, std::max(1, c->cfg.refill));
     A.tile = std::max(1, c->cfg.tile);
     A.order = nullptr;
+    // the ring order's lead launch: one camera, one frequency (its re-trace
+    // at the gate runs inside the trace kernels), the split launch in one
+    // chunk, not for the no-allocation entry point
+    const long long npix_ll = (long long)P.npix;
+    const int pct_req = noalloc ? RT_APPROACH_NONE
+                        : c->cfg.approach != 0 ? c->cfg.approach
+                                               : (npix_ll * ncam >= kApproachRays ? 100
+                                                                                  : RT_APPROACH_NONE);
+    const bool ring = c->cfg.order == RT_ORDER_RING && !noalloc && ncam == 1 && nf == 1 &&
+                      pct_req == 100 && npix_ll <= kMaxChunk;
+    int nlead = 0;
     if (c->cfg.order != RT_ORDER_SCAN) {
+        const rt_ctx::OrderTable *t = nullptr;
         if (noalloc) {
-            long long key[5];
-            order_key(P.width, P.height, A.tile, P.cam_x / P.width, P.cam_y / P.height, key);
-            A.order = find_order(c, key);
+            long long key[kOrderKey];
+            order_key(P.width, P.height, A.tile, P.cam_x / P.width, P.cam_y / P.height, false, 0.,
+                      0., key);
+            t = find_order(c, key);
         } else {
             rc = ensure_order(c, P.width, P.height, A.tile, P.cam_x / P.width,
-                              P.cam_y / P.height, &A.order);
+                              P.cam_y / P.height, ring, P.a, incl[0], &t);
             if (rc)
                 return rc;
+        }
+        if (t) {
+            A.order = t->d;
+            nlead = ring ? t->nlead : 0;
         }
     }
     if (A.order) {
@@ -1203,19 +1338,37 @@ static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *fr
     const int pct = noalloc ? RT_APPROACH_NONE
                     : c->cfg.approach != 0 ? c->cfg.approach
                                            : (total >= kApproachRays ? 100 : RT_APPROACH_NONE);
+    // ring: the lead launch takes the table's first nlead pixels, the split
+    // launch the rest
+    TraceArgs L = A;
+    if (nlead > 0) {
+        L.order = A.order;
+        L.per_cam = nlead;
+        L.total = nlead;
+        L.w0 = 0;
+        L.nsaved = 0;
+        L.st = RayState{};
+        L.queue = c->d_queue + 2;
+        L.xcount = L.queue + 1;
+        A.order += nlead;
+        A.per_cam = P.npix - nlead;
+    }
+    const int total_main = ncam * A.per_cam;
     if (pct > 0 &&
-        (rc = ensure_state(c, (std::min<long long>(total, kMaxChunk) * pct + 99) / 100, &A.st)))
+        (rc = ensure_state(c, (std::min<long long>(total_main, kMaxChunk) * pct + 99) / 100,
+                           &A.st)))
         return rc;
     if (!c->capturing)
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     hipError_t e;
     bool d3 = c->grid.n3 > 1;
+    const TraceArgs *lead = nlead > 0 ? &L : nullptr;
     if (grad)
-        e = d3 ? dispatch_int<true, true>(c, P, A, total, pct)
-               : dispatch_int<true, false>(c, P, A, total, pct);
+        e = d3 ? dispatch_int<true, true>(c, P, A, total_main, pct, lead)
+               : dispatch_int<true, false>(c, P, A, total_main, pct, lead);
     else
-        e = d3 ? dispatch_int<false, true>(c, P, A, total, pct)
-               : dispatch_int<false, false>(c, P, A, total, pct);
+        e = d3 ? dispatch_int<false, true>(c, P, A, total_main, pct, lead)
+               : dispatch_int<false, false>(c, P, A, total_main, pct, lead);
     HIPCHK(c, e);
     if (d_spectrum) {
         const int nb = (int)std::min<long long>(kSpecBlocks, ((long long)P.npix + 1023) / 1024);
@@ -1311,7 +1464,11 @@ rt_ctx *rt_create(int device)
         (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess ||
         (e = hipEventCreate(&c->evA)) != hipSuccess || (e = hipEventCreate(&c->evT)) != hipSuccess ||
         (e = hipEventCreate(&c->evX)) != hipSuccess ||
-        (e = hipMalloc((void **)&c->d_queue, 2 * sizeof(unsigned int))) != hipSuccess) {
+        (e = hipStreamCreateWithFlags(&c->lead_stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreate(&c->evL0)) != hipSuccess || (e = hipEventCreate(&c->evL1)) != hipSuccess ||
+        (e = hipMalloc((void **)&c->d_queue, 4 * sizeof(unsigned int))) != hipSuccess) {
         fail(nullptr, RT_E_HIP, "HIP set-up failed: %s", hipGetErrorString(e));
         rt_destroy(c);
         return nullptr;
@@ -1369,9 +1526,13 @@ void rt_destroy(rt_ctx *c)
         (void)hipEventDestroy(c->ev0);
     if (c->ev1)
         (void)hipEventDestroy(c->ev1);
-    for (hipEvent_t ev : {c->evA, c->evT, c->evX})
+    for (hipEvent_t ev : {c->evA, c->evT, c->evX, c->ev_fork, c->ev_join, c->evL0, c->evL1})
         if (ev)
             (void)hipEventDestroy(ev);
+    if (c->lead_stream) {
+        (void)hipStreamSynchronize(c->lead_stream);
+        (void)hipStreamDestroy(c->lead_stream);
+    }
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1532,6 +1693,12 @@ int rt_last_kernel_times(rt_ctx *c, double *out)
         HIPCHK(c, hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
         out[i] = ms;
     }
+    out[4] = 0.;
+    if (c->lead_timed) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->evL0, c->evL1));
+        out[4] = ms;
+    }
     return RT_OK;
 }
 
@@ -1676,16 +1843,18 @@ int rt_set_launch_config(rt_ctx *c, const rt_launch_config *cfg)
     if (!cfg || (cfg->block != 0 && cfg->block != 64) || cfg->blocks_per_cu < 0 ||
         cfg->blocks_per_cu > 64 || cfg->refill < 0 || cfg->refill > 64 || cfg->tile < 0 ||
         cfg->tile > 64 ||
-        (cfg->order != 0 && cfg->order != RT_ORDER_CENTRE && cfg->order != RT_ORDER_SCAN) ||
+        (cfg->order != 0 && cfg->order != RT_ORDER_CENTRE && cfg->order != RT_ORDER_SCAN &&
+         cfg->order != RT_ORDER_RING) ||
         cfg->approach < RT_APPROACH_NONE || cfg->approach > 100)
         return fail(c, RT_E_INVALID, "bad launch config (block 64, blocks_per_cu 0..64, "
-                                     "refill 0..64, tile 0..64, order 0, RT_ORDER_CENTRE or "
-                                     "RT_ORDER_SCAN, approach RT_APPROACH_NONE or 0..100)");
+                                     "refill 0..64, tile 0..64, order 0, RT_ORDER_CENTRE, "
+                                     "RT_ORDER_SCAN or RT_ORDER_RING, approach RT_APPROACH_NONE "
+                                     "or 0..100)");
     c->cfg.block = 64;
     c->cfg.blocks_per_cu = cfg->blocks_per_cu;
     c->cfg.refill = cfg->refill == 0 ? kDefaultRefill : cfg->refill;
     c->cfg.tile = cfg->tile == 0 ? kDefaultTile : cfg->tile;
-    c->cfg.order = cfg->order == 0 ? RT_ORDER_CENTRE : cfg->order;
+    c->cfg.order = cfg->order == 0 ? RT_ORDER_RING : cfg->order;
     c->cfg.approach = cfg->approach;
     return RT_OK;
 }

@@ -59,6 +59,7 @@ EMIS_THSYNCH, EMIS_THSYNCHAV, EMIS_KAPPA = 0, 1, 2
 # claim orders of rt_launch_config.order (include/raptor_rt.h)
 ORDER_CENTRE = 1
 ORDER_SCAN = 2
+ORDER_RING = 3
 
 # rt_launch_config.approach: percent of the rays whose camera-to-grid leg runs
 # in a kernel of its own (1..100), or none
@@ -200,7 +201,7 @@ class Tracer:
                                            prims.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
 
     def set_launch_config(self, blocks_per_cu=0, refill=0, tile=0, order=0, approach=0):
-        """order: ORDER_CENTRE, ORDER_SCAN or 0 (library default, centre);
+        """order: ORDER_RING, ORDER_CENTRE, ORDER_SCAN or 0 (library default, ring);
         approach: percent of the rays (in claim order) whose camera-to-grid leg
         runs in a kernel of its own, 1..100, APPROACH_NONE or 0 (library
         default)."""
@@ -260,10 +261,11 @@ class Tracer:
 
     def last_kernel_times(self):
         """ms of the last render's approach, trace, FP64 re-trace and spectrum
-        kernels (renders of one launch chunk)."""
-        out = np.zeros(4)
+        kernels and of the ring order's lead kernel, which runs beside them
+        (renders of one launch chunk)."""
+        out = np.zeros(5)
         self._check(self.lib.rt_last_kernel_times(self.ctx, _d(out)))
-        return dict(zip(("approach", "trace", "exact", "spectrum"), out))
+        return dict(zip(("approach", "trace", "exact", "spectrum", "lead"), out))
 
     def jansky_factor(self):
         return self.lib.rt_jansky_factor(self.ctx)

@@ -57,13 +57,13 @@ def test_bad_launch_config_rejected(tracer):
     with pytest.raises(rt.RtError):
         tracer.set_launch_config(tile=-1)
     with pytest.raises(rt.RtError):
-        tracer.set_launch_config(order=3)
+        tracer.set_launch_config(order=4)
     with pytest.raises(rt.RtError):
         tracer.set_launch_config(approach=101)
     with pytest.raises(rt.RtError):
         tracer.set_launch_config(approach=-2)
     cfg = tracer.get_launch_config()
-    assert cfg["order"] == rt.ORDER_CENTRE and cfg["approach"] == 0
+    assert cfg["order"] == rt.ORDER_RING and cfg["approach"] == 0
 
 
 def test_split_equals_fused_with_gradients_and_counters(tracer):
@@ -235,6 +235,68 @@ def test_graph_replay_matches_eager(tracer):
     finally:
         for d in (d1, d2, dx):
             tracer.free(d)
+
+
+def test_ring_order_lead_launch_is_bit_identical(tracer):
+    """The ring order traces the pixels near the critical curve from the camera
+    in a lead kernel on a second stream while the split launch takes the
+    rest: same image, gradient and per-ray counters as the centre order,
+    at 1024^2 (the default split launch) and two inclinations (the curve
+    moves with the inclination), and the lead kernel ran."""
+    W = 1024
+    _params(tracer, W)
+    n = W * W
+    d = tracer.alloc(n * 8)
+    dd = tracer.alloc(n * 16)
+    dc = tracer.alloc(n * 8)
+    try:
+        for incl in (INCL, 17.0):
+            out = {}
+            for order in (rt.ORDER_CENTRE, rt.ORDER_RING):
+                tracer.set_launch_config(order=order)
+                tracer.render_device([incl], [FREQ], d, dd, dc)
+                tracer.synchronize()
+                lead_ms = tracer.last_kernel_times()["lead"]
+                out[order] = (tracer.d2h(d, (n,), np.float64), tracer.d2h(dd, (n, 2), np.float64),
+                              tracer.d2h(dc, (n, 2), np.int32), lead_ms)
+            for k in range(3):
+                assert np.array_equal(out[rt.ORDER_CENTRE][k], out[rt.ORDER_RING][k]), (incl, k)
+            assert out[rt.ORDER_CENTRE][3] == 0.0 and out[rt.ORDER_RING][3] > 0.0
+    finally:
+        tracer.set_launch_config()
+        for p in (d, dd, dc):
+            tracer.free(p)
+
+
+def test_graph_replays_the_ring_order_lead_launch(tracer):
+    """A graph captured around a ring-order render forks the lead kernel onto
+    the second stream and joins it back: every replay, with eager renders in
+    between, equals the eager image."""
+    W = 1024
+    _params(tracer, W)
+    n = W * W
+    d = tracer.alloc(n * 8)
+    dx = tracer.alloc(n * 8)
+    try:
+        tracer.render_device([INCL], [FREQ], d)
+        tracer.synchronize()
+        e = tracer.d2h(d, (n,), np.float64)
+        tracer.capture_begin()
+        tracer.render_device([INCL], [FREQ], d)
+        g = tracer.capture_end()
+        try:
+            for k in range(3):
+                tracer.render_device([80.0], [FREQ], dx)
+                tracer.synchronize()
+                tracer.d2h(dx, (n,), np.float64)
+                g.launch()
+                tracer.synchronize()
+                assert np.array_equal(tracer.d2h(d, (n,), np.float64), e), k
+        finally:
+            g.destroy()
+    finally:
+        tracer.free(d)
+        tracer.free(dx)
 
 
 def test_graph_keeps_its_claim_order_after_other_shapes(tracer):
