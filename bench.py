@@ -146,12 +146,15 @@ def _counters(tr, incl, freqs, grad=False):
         tr.free(d_cnt)
 
 
-def _rooflines(cnt, nf, kms, per, grid_step=None, grad=False, kernel="trace kernel"):
+def _rooflines(cnt, nf, kms, per, grid_step=None, grad=False, kernel="trace kernel",
+               lead=None):
     """Roofline of the render's kernels together and, given the rays' grid
-    entry steps (the oracle's counters) with the kernels' own steps of the
-    same rays (per ray, or a sample: then the approach share of the steps is
-    scaled to the image), of the trace kernel and the approach kernel on
-    their own."""
+    entry steps (the oracle's counters: gs for the sampled pixels pix, whose
+    own step counts are st), of the trace kernel, the approach kernel and
+    the ring order's lead kernel (lead: mask of the rays it traces, all their
+    steps and samples) on their own.  With every ray sampled the approach /
+    trace split of the other rays is exact; with a sample, the approach share
+    of their steps is the sample's."""
     from raptor_amd import flops
     f64, f32 = flops.image_flops_split(cnt[:, 0], cnt[:, 1], nf, grad=grad)
     out = {"render": _roof(f64, f32, kms, "all kernels of the render"),
@@ -159,19 +162,38 @@ def _rooflines(cnt, nf, kms, per, grid_step=None, grad=False, kernel="trace kern
            "samples_per_launch": int(cnt[:, 1].sum())}
     if per is None or grid_step is None:
         return out
-    gs, st = grid_step
-    if gs.shape[0] == cnt.shape[0]:
-        (a64, _), (t64, t32) = flops.kernel_split(cnt[:, 0], gs, cnt[:, 1], nf, grad=grad)
+    gs, st, pix = grid_step
+    main = np.ones(cnt.shape[0], bool) if lead is None else ~lead
+    if lead is not None and lead.any():
+        l64, l32 = flops.image_flops_split(cnt[lead, 0], cnt[lead, 1], nf, grad=grad)
+        out["lead"] = _roof(l64, l32, per["lead"], "trace_kernel<1,...,false,0> (lead)")
+        out["lead_rays"] = int(lead.sum())
+    m64, m32 = flops.image_flops_split(cnt[main, 0], cnt[main, 1], nf, grad=grad)
+    sm = main[pix]
+    if pix.size == cnt.shape[0]:
+        (a64, _), (t64, t32) = flops.kernel_split(cnt[main, 0], gs[sm], cnt[main, 1], nf,
+                                                  grad=grad)
         out["split_from"] = "per-ray grid entry steps of the oracle (all rays)"
     else:
-        share = float(np.where(gs >= 0, gs, st).sum()) / float(st.sum())
-        a64 = share * f64
-        t64, t32 = f64 - a64, f32
+        share = float(np.where(gs[sm] >= 0, gs[sm], st[sm]).sum()) / float(st[sm].sum())
+        a64 = share * m64
+        t64, t32 = m64 - a64, m32
         out["split_from"] = ("approach share of the steps %.4f from the oracle's sample of "
-                             "%d rays" % (share, gs.shape[0]))
+                             "%d rays" % (share, int(sm.sum())))
     out["trace"] = _roof(t64, t32, per["trace"], kernel)
     out["approach"] = _roof(a64, 0.0, per["approach"], "approach_kernel<0>")
     return out
+
+
+def _lead_mask(tr, incl, per):
+    """Rays the ring order's lead kernel traced in the last render (None if it
+    did not run)."""
+    if not per or per.get("lead", 0.0) <= 0.0:
+        return None
+    order, nl = tr.claim_order(incl)
+    m = np.zeros(order.size, bool)
+    m[order[:nl]] = True
+    return m
 
 
 def bench_c1(args):
@@ -219,6 +241,7 @@ def bench_c1(args):
     jy = tr.jansky_factor()
     launch = tr.get_launch_config()
     split = launch["approach"] > 0 or (launch["approach"] == 0 and npix >= 1 << 20)
+    lead = _lead_mask(tr, incl[0], per)
     trace_name = ("trace_kernel_3w<1,false,false,0,%s,0>" % ("true" if split else "false"))
 
     result = {
@@ -263,9 +286,10 @@ def bench_c1(args):
         agree = np.where(op >= 0, (st > op) & (st <= ocnt[:, 0]), st == ocnt[:, 0])
         result["parity"]["steps_agree_oracle_frac"] = float(np.mean(agree))
         result["parity"]["opaque_frac"] = float(np.mean(op >= 0))
-        grid_step = (ocnt[:, 4], cnt[pix, 0])
+        grid_step = (ocnt[:, 4], cnt[pix, 0], pix)
 
-    roof = _rooflines(cnt, 1, kms, per, grid_step, kernel=trace_name) if thermal else None
+    roof = (_rooflines(cnt, 1, kms, per, grid_step, kernel=trace_name, lead=lead)
+            if thermal else None)
     traffic = issue = None
     prof = {}
     tfile = os.path.join(ROOT, "profiles", "trace_kernel_traffic.json")
@@ -290,12 +314,15 @@ def bench_c1(args):
             "render_traffic_algorithmic": prof.get("render_algorithmic_bytes"),
             "kernel": t["kernel"], "kernel_ms": t["kernel_ms"],
             "kernel_note": "the dominant kernel alone, HIP events around it on the library "
-                           "stream; algorithmic work = its RK2 steps (from the grid-entry "
-                           "step on) x 566 FP64 + samples x 300 FP32",
+                           "stream; algorithmic work = its rays' RK2 steps (from the "
+                           "grid-entry step on) x 566 FP64 + samples x 300 FP32; the rays "
+                           "near the critical curve are the lead kernel's (second stream, "
+                           "beside the approach kernel)",
             "valu_issue_frac": issue,
             "peak_note": "FP64/FP32 mix-weighted vector peak (78.6 / 157.3 TFLOP/s)",
             "alg_flops_fp64": t["alg_flops_fp64"], "alg_flops_fp32": t["alg_flops_fp32"],
-            "render": roof["render"], "approach": roof["approach"],
+            "render": roof["render"], "approach": roof["approach"], "lead": roof.get("lead"),
+            "lead_rays": roof.get("lead_rays"),
             "split_from": roof["split_from"],
             "rk2_steps_per_launch": roof["rk2_steps_per_launch"],
             "samples_per_launch": roof["samples_per_launch"],
@@ -441,6 +468,7 @@ def bench_3d(args, which):
            "image": {"kernel_ms": ki, "kernels": peri, "wall_ms": wi,
                      "images_per_s": 1e3 / wi}}
     img = tr.d2h(d_img, (npix,), np.float64)
+    lead = _lead_mask(tr, incl[0], peri)
     if which == "c4":
         tr.render_device(incl, freq, d_img, d_dimg, d_spectrum=d_spec)
         kg, perg, wg = _timed(
@@ -474,11 +502,11 @@ def bench_3d(args, which):
                       "%.1f s wall" % (stride, pix.size, c1 - c0)}
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
         out["parity"] = _parity(img[pix], ora0[:, 0], gate0[:, 0])
-        grid_step = (ocnt[:, 4], cnt[pix, 0])
+        grid_step = (ocnt[:, 4], cnt[pix, 0], pix)
     split = npix >= 1 << 20
     out["image"]["roofline"] = _rooflines(
         cnt, 1, ki, peri if split else None, grid_step if split else None,
-        kernel="trace_kernel_3w<1,false,true,0,true,0>")
+        kernel="trace_kernel_3w<1,false,true,0,true,0>", lead=lead)
     if which == "c4":
         out["grad"]["roofline"] = _rooflines(gcnt, 1, kg, None, grad=True)["render"]
     if which == "c4" and args.cpu_sample > 0:

@@ -219,6 +219,15 @@ double rt_jansky_factor(const rt_ctx *ctx);
 #define RT_ORDER_CENTRE 1
 #define RT_ORDER_SCAN 2
 #define RT_ORDER_RING 3
+
+/* The claim order that a render of one camera at `inclination` (degrees)
+ * and one frequency uses with the current image parameters and launch
+ * configuration: h_order[W*H] pixel indices (y*W + x) in the order their
+ * rays are claimed, of which the first *n_lead are traced by the ring
+ * order's lead kernel (0 when it does not run).  Builds the table if
+ * needed, as an eager render would.  No counterpart in the reference, whose pixel loop runs in index
+ * order (core.c:107-113): for measurement and tests. */
+int rt_claim_order(rt_ctx *ctx, double inclination, int *h_order, int *n_lead);
 #define RT_APPROACH_NONE (-1)
 typedef struct {
     int block;

@@ -734,7 +734,7 @@ struct rt_ctx {
     float ktab_u0 = 0.f, ktab_idu = 1.f;
     int ktab_n = 0;
     double last_ms = 0.;
-    rt_launch_config cfg{64, 0, kDefaultRefill, kDefaultTile, RT_ORDER_CENTRE, 0};
+    rt_launch_config cfg{64, 0, kDefaultRefill, kDefaultTile, RT_ORDER_RING, 0};
     bool capturing = false;   // between rt_capture_begin and rt_capture_end
     char err[512] = {0};
 };
@@ -1241,6 +1241,18 @@ static int ensure_order(rt_ctx *c, int W, int H, int T, double sx, double sy, bo
     return RT_OK;
 }
 
+// Whether a render of ncam cameras and nf frequencies over npix pixels
+// takes the ring order's lead launch: the ring order, one camera, one
+// frequency (its re-trace at the gate runs inside the trace kernels), and
+// the split launch of all rays in one chunk.
+static bool ring_launch(const rt_ctx *c, long long npix, int ncam, int nf)
+{
+    const int pct = c->cfg.approach != 0 ? c->cfg.approach
+                                         : (npix * ncam >= kApproachRays ? 100 : RT_APPROACH_NONE);
+    return c->cfg.order == RT_ORDER_RING && ncam == 1 && nf == 1 && pct == 100 &&
+           npix <= kMaxChunk;
+}
+
 // noalloc (rt_render_device_noalloc): the work queue, the FP64 re-trace's
 // list and the spectrum's partial sums in the device's `scratch`; one fused
 // launch of at most kNoAllocRays work items; a centre-out claim order only
@@ -1274,13 +1286,7 @@ static int render_impl(rt_ctx *c, const double *incl, int ncam, const double *fr
     // the ring order's lead launch: one camera, one frequency (its re-trace
     // at the gate runs inside the trace kernels), the split launch in one
     // chunk, not for the no-allocation entry point
-    const long long npix_ll = (long long)P.npix;
-    const int pct_req = noalloc ? RT_APPROACH_NONE
-                        : c->cfg.approach != 0 ? c->cfg.approach
-                                               : (npix_ll * ncam >= kApproachRays ? 100
-                                                                                  : RT_APPROACH_NONE);
-    const bool ring = c->cfg.order == RT_ORDER_RING && !noalloc && ncam == 1 && nf == 1 &&
-                      pct_req == 100 && npix_ll <= kMaxChunk;
+    const bool ring = !noalloc && ring_launch(c, P.npix, ncam, nf);
     int nlead = 0;
     if (c->cfg.order != RT_ORDER_SCAN) {
         const rt_ctx::OrderTable *t = nullptr;
@@ -1833,6 +1839,48 @@ int rt_graph_destroy(rt_ctx *c, rt_graph *g)
         }
     }
     delete g;
+    return RT_OK;
+}
+
+int rt_claim_order(rt_ctx *c, double inclination, int *h_order, int *n_lead)
+{
+    if (!c)
+        return RT_E_INVALID;
+    if (int rc0 = no_capture(c, "rt_claim_order"))
+        return rc0;
+    if (!h_order || !n_lead)
+        return fail(c, RT_E_INVALID, "output buffers must not be NULL");
+    Params P;
+    const double freq = 2.3e11;
+    int rc = make_params(c, &inclination, 1, &freq, 1, P);
+    if (rc)
+        return rc;
+    if ((rc = select_device(c)))
+        return rc;
+    const size_t n = (size_t)P.npix;
+    *n_lead = 0;
+    if (c->cfg.order == RT_ORDER_SCAN) {
+        // work_to_ray's scan: T x T tiles in row-major tile order, pixels
+        // row-major inside a tile, items outside the image skipped
+        const int T = std::max(1, c->cfg.tile), W = P.width, H = P.height;
+        const int ntx = (W + T - 1) / T, nty = (H + T - 1) / T;
+        size_t k = 0;
+        for (int t = 0; t < ntx * nty; t++)
+            for (int l = 0; l < T * T; l++) {
+                const int x = (t % ntx) * T + l % T, y = (t / ntx) * T + l / T;
+                if (x < W && y < H)
+                    h_order[k++] = y * W + x;
+            }
+        return RT_OK;
+    }
+    const bool ring = ring_launch(c, P.npix, 1, 1);
+    const rt_ctx::OrderTable *t = nullptr;
+    if ((rc = ensure_order(c, P.width, P.height, std::max(1, c->cfg.tile), P.cam_x / P.width,
+                           P.cam_y / P.height, ring, P.a, inclination, &t)))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(h_order, t->d, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *n_lead = ring ? t->nlead : 0;
     return RT_OK;
 }
 

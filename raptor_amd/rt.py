@@ -90,6 +90,8 @@ SIGNATURES = [
     ("rt_last_kernel_times", ctypes.c_int, [_vp, _dp]),
     ("rt_jansky_factor", ctypes.c_double, [_vp]),
     ("rt_set_launch_config", ctypes.c_int, [_vp, ctypes.POINTER(RtLaunchConfig)]),
+    ("rt_claim_order", ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_void_p,
+                                      ctypes.POINTER(ctypes.c_int)]),
     ("rt_get_launch_config", ctypes.c_int, [_vp, ctypes.POINTER(RtLaunchConfig)]),
     ("rt_capture_begin", ctypes.c_int, [_vp]),
     ("rt_capture_end", ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
@@ -208,6 +210,16 @@ class Tracer:
         cfg = RtLaunchConfig(block=64, blocks_per_cu=blocks_per_cu, refill=refill, tile=tile,
                              order=order, approach=approach)
         self._check(self.lib.rt_set_launch_config(self.ctx, ctypes.byref(cfg)))
+
+    def claim_order(self, inclination):
+        """(order, n_lead): the pixels in the order a one-camera, one-frequency
+        render at this inclination claims them, the first n_lead traced by the
+        ring order's lead kernel."""
+        order = np.zeros(self.width * self.height, np.int32)
+        nl = ctypes.c_int(0)
+        self._check(self.lib.rt_claim_order(self.ctx, float(inclination), order.ctypes.data,
+                                            ctypes.byref(nl)))
+        return order, nl.value
 
     def get_launch_config(self):
         cfg = RtLaunchConfig()

@@ -242,7 +242,10 @@ def test_ring_order_lead_launch_is_bit_identical(tracer):
     in a lead kernel on a second stream while the split launch takes the
     rest: same image, gradient and per-ray counters as the centre order,
     at 1024^2 (the default split launch) and two inclinations (the curve
-    moves with the inclination), and the lead kernel ran."""
+    moves with the inclination), and the lead kernel ran.  A new context
+    starts with the ring order."""
+    with rt.Tracer(0) as fresh:
+        assert fresh.get_launch_config()["order"] == rt.ORDER_RING
     W = 1024
     _params(tracer, W)
     n = W * W
@@ -266,6 +269,63 @@ def test_ring_order_lead_launch_is_bit_identical(tracer):
         tracer.set_launch_config()
         for p in (d, dd, dc):
             tracer.free(p)
+
+
+def _critical_curve(a, incl_deg, n=200001, ds=0.002):
+    """Bardeen's critical curve (impact parameters alpha, beta of the
+    spherical photon orbits) for a camera at incl_deg around spin a, with
+    this code's camera convention alpha = -xi / sin i, as points every ds
+    (M) of arc length (even spacing keeps the k-d tree below fast)."""
+    inc = np.radians(incl_deg)
+    r1 = 2 * (1 + np.cos(2 / 3 * np.arccos(-a)))
+    r2 = 2 * (1 + np.cos(2 / 3 * np.arccos(a)))
+    r = np.linspace(min(r1, r2), max(r1, r2), n)
+    xi = (r**2 * (3 - r) - a**2 * (r + 1)) / (a * (r - 1))
+    eta = r**3 * (4 * a**2 - r * (r - 3)**2) / (a**2 * (r - 1)**2)
+    b2 = eta + a**2 * np.cos(inc)**2 - xi**2 / np.tan(inc)**2
+    ok = b2 >= 0
+    al, be = -xi[ok] / np.sin(inc), np.sqrt(b2[ok])
+    A, B = np.r_[al, al[::-1]], np.r_[be, -be[::-1]]   # upper branch, lower reversed
+    s = np.r_[0.0, np.cumsum(np.hypot(np.diff(A), np.diff(B)))]
+    u = np.arange(0.0, s[-1], ds)
+    return np.interp(u, s, A), np.interp(u, s, B)
+
+
+def test_ring_order_leads_with_the_pixels_at_the_critical_curve(tracer, dump128):
+    """rt_claim_order: the ring order is a permutation of the pixels whose first
+    n_lead are those within 0.3 M of the critical curve (here from an
+    independent dense sampling, with a margin for the library's sampling)
+    and the rest the centre order without them; the centre order has no
+    lead; images below the split launch's size have none either.  The curve
+    moves with the inclination, and its pixels are where the longest rays
+    are (the photon ring)."""
+    from scipy.spatial import cKDTree
+    W = 1024
+    _params(tracer, W)
+    fov = 40.0
+    yy, xx = np.divmod(np.arange(W * W), W)
+    al = -fov / 2 + (xx + 0.5) * fov / W
+    be = -fov / 2 + (yy + 0.5) * fov / W
+    try:
+        for incl in (INCL, 17.0, 150.0):
+            ring, nl = tracer.claim_order(incl)
+            assert np.array_equal(np.sort(ring), np.arange(W * W))
+            ca, cb = _critical_curve(dump128.grid.a, incl)
+            d, _ = cKDTree(np.c_[ca, cb]).query(np.c_[al, be], distance_upper_bound=0.4)
+            lead = np.zeros(W * W, bool)
+            lead[ring[:nl]] = True
+            assert np.all(lead[d < 0.3 - 0.01]) and not np.any(lead[d > 0.3 + 0.01]), incl
+            assert nl > 5000
+            tracer.set_launch_config(order=rt.ORDER_CENTRE)
+            centre, nc = tracer.claim_order(incl)
+            tracer.set_launch_config()
+            assert nc == 0
+            assert np.array_equal(ring[nl:], centre[~lead[centre]])
+            assert np.array_equal(ring[:nl], centre[lead[centre]])
+        _params(tracer, 512)
+        assert tracer.claim_order(INCL)[1] == 0
+    finally:
+        tracer.set_launch_config()
 
 
 def test_graph_replays_the_ring_order_lead_launch(tracer):
