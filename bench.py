@@ -316,8 +316,8 @@ def bench_c1(args):
             "kernel_note": "the dominant kernel alone, HIP events around it on the library "
                            "stream; algorithmic work = its rays' RK2 steps (from the "
                            "grid-entry step on) x 566 FP64 + samples x 300 FP32; the rays "
-                           "near the critical curve are the lead kernel's (second stream, "
-                           "beside the approach kernel)",
+                           "near the critical curve or the spin axis are the lead kernel's "
+                           "(second stream, beside the approach kernel)",
             "valu_issue_frac": issue,
             "peak_note": "FP64/FP32 mix-weighted vector peak (78.6 / 157.3 TFLOP/s)",
             "alg_flops_fp64": t["alg_flops_fp64"], "alg_flops_fp32": t["alg_flops_fp32"],

@@ -1060,15 +1060,19 @@ static int ensure_state(rt_ctx *c, long long slots, RayState *out)
 // not end waiting on a few long rays that started late.  Built on the host
 // once per image size and tile, kept in HBM (4 bytes per pixel).
 //
-// Ring order (one camera, split launch): the same, with the pixels within
+// Ring order (one camera, split launch): the same, with the lead set moved
+// to the front, in centre-out order among themselves: the pixels within
 // kRingWidth of the camera's critical curve (Bardeen's photon-orbit curve
-// for the spin and inclination) moved to the front, in centre-out order
-// among themselves.  Those are the longest rays (they orbit the hole; where
-// the curve crosses the projected spin axis they also pass the pole, where
-// steps shrink with min(X2, 1 - X2)): the lead kernel traces them from the
-// camera on a second stream from the start of the render, beside the
-// approach kernel, instead of after it.
+// for the spin and inclination), whose rays orbit the hole, and the polar
+// band, whose rays turn in theta within arcsin(kPolarSin) of the spin axis,
+// where the step-size rule (integrator.c:109-122) shrinks the steps with
+// min(X2, 1 - X2) (at 1024^2, a = 0.9375, i = 60: up to 1300 steps inside
+// the grid against a mean of 200, in a column of pixels from the shadow to
+// the image's edge, the last of which the centre-out order claims last).
+// The lead kernel traces them from the camera on a second stream from the
+// start of the render, beside the approach kernel, instead of after it.
 static constexpr double kRingWidth = 0.3;   // M: 12304 of 1024^2 pixels at 40 M, a = 0.9375, i = 60
+static constexpr double kPolarSin = 0.02;   // 12022 more pixels there
 
 static const rt_ctx::OrderTable *find_order(rt_ctx *c, const long long key[kOrderKey])
 {
@@ -1101,11 +1105,37 @@ static void order_key(int W, int H, int T, double sx, double sy, bool ring, doub
 // (a circle of radius sqrt(27) for a = 0), sampled densely and each sample
 // marking the pixel centres within `width` of it.  Pixel centres as
 // pixel_impact: alpha = -cam_x/2 + (x + 1/2) sx, beta = -cam_y/2 + (y + 1/2) sy.
+// With polar > 0 also the pixels whose ray turns in theta closer to the axis
+// than sin theta_min = polar: with the camera's constants of motion (E = 1,
+// L = -alpha sin i, Carter Q = beta^2 + cos^2 i (alpha^2 - a^2)), the theta
+// potential Q + a^2 cos^2 - L^2 cot^2 vanishes at u = cos^2 theta_min, the
+// root in [0, 1] of a^2 u^2 + (Q + L^2 - a^2) u - Q = 0 (u = Q / (Q + L^2)
+// for a = 0); rays with Q <= 0 never leave the equatorial band.
 static void ring_mask(int W, int H, double sx, double sy, double a, double incl,
-                      double width, std::vector<char> &mask)
+                      double width, double polar, std::vector<char> &mask)
 {
     mask.assign((size_t)W * H, 0);
     const double si = std::sin(incl * kPi / 180.), ci = std::cos(incl * kPi / 180.);
+    for (int y = 0; polar > 0. && y < H; y++) {
+        const double be = -0.5 * H * sy + (y + 0.5) * sy;
+        for (int x = 0; x < W; x++) {
+            const double al = -0.5 * W * sx + (x + 0.5) * sx;
+            const double L = -al * si, Q = be * be + ci * ci * (al * al - a * a);
+            if (!(Q > 0.))
+                continue;
+            double u;
+            if (a * a < 1e-12) {
+                u = Q / (Q + L * L);
+            } else {
+                // the root without cancellation: 2Q / (B + sqrt(B^2 + 4 a^2 Q))
+                const double B = Q + L * L - a * a;
+                const double sq = std::sqrt(B * B + 4. * a * a * Q);
+                u = B > 0. ? 2. * Q / (B + sq) : (sq - B) / (2. * a * a);
+            }
+            if (1. - u < polar * polar)
+                mask[(size_t)y * W + x] = 1;
+        }
+    }
     auto mark = [&](double al, double be) {
         const int x0 = std::max(0, (int)std::floor((al - width + 0.5 * W * sx) / sx - 0.5));
         const int x1 = std::min(W - 1, (int)std::ceil((al + width + 0.5 * W * sx) / sx - 0.5));
@@ -1183,7 +1213,7 @@ static int ensure_order(rt_ctx *c, int W, int H, int T, double sx, double sy, bo
         order.resize(n);
         c->orders.reserve(c->orders.size() + 1);
         if (ring)
-            ring_mask(W, H, sx, sy, a, incl, kRingWidth, lead);
+            ring_mask(W, H, sx, sy, a, incl, kRingWidth, kPolarSin, lead);
     } catch (const std::bad_alloc &) {
         return fail(c, RT_E_NOMEM, "out of host memory for the claim order");
     }

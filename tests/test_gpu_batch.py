@@ -3,11 +3,13 @@
 A ray's arithmetic does not depend on which lane, wave or launch traces it,
 so every launch shape must give bit-identical images, and a batched launch
 over cameras x frequencies must equal the per-image launches."""
+import warnings
+
 import numpy as np
 import pytest
 
 from conftest import FREQ, INCL, M_UNIT, R_HIGH
-from raptor_amd import autotune, rt
+from raptor_amd import autotune, rt, torus
 
 pytestmark = pytest.mark.gpu
 
@@ -276,30 +278,67 @@ def _critical_curve(a, incl_deg, n=200001, ds=0.002):
     spherical photon orbits) for a camera at incl_deg around spin a, with
     this code's camera convention alpha = -xi / sin i, as points every ds
     (M) of arc length (even spacing keeps the k-d tree below fast)."""
-    inc = np.radians(incl_deg)
-    r1 = 2 * (1 + np.cos(2 / 3 * np.arccos(-a)))
-    r2 = 2 * (1 + np.cos(2 / 3 * np.arccos(a)))
-    r = np.linspace(min(r1, r2), max(r1, r2), n)
-    xi = (r**2 * (3 - r) - a**2 * (r + 1)) / (a * (r - 1))
-    eta = r**3 * (4 * a**2 - r * (r - 3)**2) / (a**2 * (r - 1)**2)
-    b2 = eta + a**2 * np.cos(inc)**2 - xi**2 / np.tan(inc)**2
-    ok = b2 >= 0
-    al, be = -xi[ok] / np.sin(inc), np.sqrt(b2[ok])
-    A, B = np.r_[al, al[::-1]], np.r_[be, -be[::-1]]   # upper branch, lower reversed
+    if a == 0.0:   # Schwarzschild: the circle of radius sqrt(27)
+        ph = np.linspace(0.0, 2 * np.pi, n)
+        A, B = np.sqrt(27.0) * np.cos(ph), np.sqrt(27.0) * np.sin(ph)
+    else:
+        inc = np.radians(incl_deg)
+        r1 = 2 * (1 + np.cos(2 / 3 * np.arccos(-a)))
+        r2 = 2 * (1 + np.cos(2 / 3 * np.arccos(a)))
+        r = np.linspace(min(r1, r2), max(r1, r2), n)
+        xi = (r**2 * (3 - r) - a**2 * (r + 1)) / (a * (r - 1))
+        eta = r**3 * (4 * a**2 - r * (r - 3)**2) / (a**2 * (r - 1)**2)
+        b2 = eta + a**2 * np.cos(inc)**2 - xi**2 / np.tan(inc)**2
+        ok = b2 >= 0
+        al, be = -xi[ok] / np.sin(inc), np.sqrt(b2[ok])
+        A, B = np.r_[al, al[::-1]], np.r_[be, -be[::-1]]   # upper branch, lower reversed
     s = np.r_[0.0, np.cumsum(np.hypot(np.diff(A), np.diff(B)))]
     u = np.arange(0.0, s[-1], ds)
     return np.interp(u, s, A), np.interp(u, s, B)
 
 
-def test_ring_order_leads_with_the_pixels_at_the_critical_curve(tracer, dump128):
+def _polar_sin_min(a, incl_deg, al, be):
+    """sin of the polar angle at which each ray (camera constants of motion
+    E = 1, L = -alpha sin i, Q = beta^2 + cos^2 i (alpha^2 - a^2)) turns in
+    theta, by bisection of the theta potential Q + a^2 cos^2 - L^2 cot^2 on
+    (0, pi/2], where it is positive at pi/2 (Q > 0) and falls to -inf at 0;
+    inf where Q <= 0 (the ray never leaves the equatorial band)."""
+    inc = np.radians(incl_deg)
+    L = -al * np.sin(inc)
+    Q = be**2 + np.cos(inc)**2 * (al**2 - a**2)
+    lo, hi = np.full(al.shape, 1e-12), np.full(al.shape, np.pi / 2)
+    for _ in range(60):
+        th = 0.5 * (lo + hi)
+        pot = Q + a**2 * np.cos(th)**2 - L**2 / np.tan(th)**2
+        neg = pot < 0
+        lo, hi = np.where(neg, th, lo), np.where(neg, hi, th)
+    return np.where(Q > 0, np.sin(hi), np.inf)
+
+
+def _expected_lead(a, incl, al, be, ring_width=0.3, polar=0.02):
+    """(surely in, surely out) masks of the lead set: within ring_width of
+    the critical curve (a margin of 0.01 M for the library's sampling of the
+    curve) or turning in theta with sin theta_min < polar (a relative margin
+    of 1e-6 for rounding)."""
+    from scipy.spatial import cKDTree
+    ca, cb = _critical_curve(a, incl)
+    d, _ = cKDTree(np.c_[ca, cb]).query(np.c_[al, be], distance_upper_bound=0.4)
+    sp = _polar_sin_min(a, incl, al, be)
+    sure_in = (d < ring_width - 0.01) | (sp < polar * (1 - 1e-6))
+    sure_out = (d > ring_width + 0.01) & (sp > polar * (1 + 1e-6))
+    return sure_in, sure_out, sp < polar
+
+
+def test_ring_order_leads_with_the_longest_rays(tracer, dump128):
     """rt_claim_order: the ring order is a permutation of the pixels whose first
     n_lead are those within 0.3 M of the critical curve (here from an
     independent dense sampling, with a margin for the library's sampling)
-    and the rest the centre order without them; the centre order has no
-    lead; images below the split launch's size have none either.  The curve
-    moves with the inclination, and its pixels are where the longest rays
-    are (the photon ring)."""
-    from scipy.spatial import cKDTree
+    or whose rays turn in theta within arcsin 0.02 of the spin axis (here
+    by bisection of the theta potential), and the rest the centre order
+    without them; the centre order has no lead; images below the split
+    launch's size have none either.  The curve moves with the inclination,
+    and its pixels are where the longest rays are (the photon ring); the
+    polar band is where the step-size rule shrinks the steps."""
     W = 1024
     _params(tracer, W)
     fov = 40.0
@@ -310,12 +349,11 @@ def test_ring_order_leads_with_the_pixels_at_the_critical_curve(tracer, dump128)
         for incl in (INCL, 17.0, 150.0):
             ring, nl = tracer.claim_order(incl)
             assert np.array_equal(np.sort(ring), np.arange(W * W))
-            ca, cb = _critical_curve(dump128.grid.a, incl)
-            d, _ = cKDTree(np.c_[ca, cb]).query(np.c_[al, be], distance_upper_bound=0.4)
+            sure_in, sure_out, polar = _expected_lead(dump128.grid.a, incl, al, be)
             lead = np.zeros(W * W, bool)
             lead[ring[:nl]] = True
-            assert np.all(lead[d < 0.3 - 0.01]) and not np.any(lead[d > 0.3 + 0.01]), incl
-            assert nl > 5000
+            assert np.all(lead[sure_in]) and not np.any(lead[sure_out]), incl
+            assert nl > 5000 and polar.sum() > 2000
             tracer.set_launch_config(order=rt.ORDER_CENTRE)
             centre, nc = tracer.claim_order(incl)
             tracer.set_launch_config()
@@ -324,8 +362,22 @@ def test_ring_order_leads_with_the_pixels_at_the_critical_curve(tracer, dump128)
             assert np.array_equal(ring[:nl], centre[lead[centre]])
         _params(tracer, 512)
         assert tracer.claim_order(INCL)[1] == 0
+        # other spins, Schwarzschild's circle included (the ring follows the
+        # loaded dump's spin)
+        _params(tracer, W)
+        for a in (0.0, -0.7, 0.5):
+            with warnings.catch_warnings():
+                warnings.simplefilter("ignore")
+                tracer.load_dump(torus.fishbone_moncrief(32, 32, a=a))
+            ring, nl = tracer.claim_order(INCL)
+            sure_in, sure_out, _ = _expected_lead(a, INCL, al, be)
+            lead = np.zeros(W * W, bool)
+            lead[ring[:nl]] = True
+            assert np.all(lead[sure_in]) and not np.any(lead[sure_out]), a
+            assert np.array_equal(np.sort(ring), np.arange(W * W)) and nl > 5000
     finally:
         tracer.set_launch_config()
+        tracer.load_dump(dump128)
 
 
 def test_graph_replays_the_ring_order_lead_launch(tracer):
