@@ -150,9 +150,17 @@ def test_c4_full_workload(tracer, dump3d_full, R_high):
         err = np.abs(f1 - an) / den
         print("c4 R_high=%g d%d: smooth %.3f, within 1e-3 %.4f, max %.3g"
               % (R_high, q, smooth.mean(), np.mean(err[smooth] < 1e-3), err[smooth].max()))
+        p = 1.0 if q == 0 else R_high
+        ipix = np.maximum(img[0, fpix, 0], 1e-6 * img[0, :, 0].max())
+        e2 = np.abs(f1 - an) / np.maximum(np.abs(an), ipix / p)
+        print("   max error %.3g of the larger of tangent and I/p over smooth pixels"
+              % e2[smooth].max())
         assert smooth.mean() > 0.9
         assert np.mean(err[smooth] < 1e-3) > 0.98 and err[smooth].max() < 2e-2
         assert f1[smooth].sum() / an[smooth].sum() == pytest.approx(1.0, abs=5e-4)
+        # every smooth pixel to 1e-3 of the larger of its tangent and I / p
+        # (a unit logarithmic derivative)
+        assert e2[smooth].max() < 1e-3, q
 
 
 def test_device_spectrum_and_kernel_times(tracer, dump128):

@@ -35,7 +35,7 @@ def test_gpu_gradient_against_oracle_differences_step_1e3(dump3d, R_high):
     with rt.Tracer(0) as tr:
         tr.load_dump(dump3d)
         tr.set_params(width=W, height=W, M_unit=M_UNIT, R_high=R_high)
-        _, dimg, _ = tr.render([INCL], [FREQ], grad=True)
+        img, dimg, _ = tr.render([INCL], [FREQ], grad=True)
     kw = dict(width=W, height=W, inclination=INCL)
 
     def ora(M, R):
@@ -58,3 +58,12 @@ def test_gpu_gradient_against_oracle_differences_step_1e3(dump3d, R_high):
         assert smooth.mean() > 0.9
         assert np.mean(err[smooth] < 1e-3) > 0.98 and err[smooth].max() < 2e-2
         assert f1[smooth].sum() / an[smooth].sum() == pytest.approx(1.0, abs=5e-4)
+        # every smooth pixel to 1e-3 of the larger of its tangent and I / p
+        # (a unit logarithmic derivative d ln I / d ln p): relative where the
+        # tangent is large, as a logarithmic derivative where it is near zero
+        p = 1.0 if q == 0 else R0
+        ipix = np.maximum(img[0, :, 0], 1e-6 * img[0, :, 0].max())
+        e2 = np.abs(f1 - an) / np.maximum(np.abs(an), ipix / p)
+        print("R_high %g d%d: max error %.3g (tangent or I/p scale) over %d smooth pixels"
+              % (R0, q, e2[smooth].max(), smooth.sum()))
+        assert e2[smooth].max() < 1e-3, q
