@@ -864,6 +864,17 @@ __host__ __device__ inline EmisPre emission_pre(float B, float mu, float Te, flo
     return p;
 }
 
+// The FP64 overflow test of the zero-emissivity case (poison) for a plasma
+// frequency nu_p at dimensionless temperature Te.
+__host__ __device__ inline bool planck_poison(float Te, float nu_p)
+{
+    const double T = (double)Te * kElectronMass * kC * kC / kBoltzmann;
+    return kPlanck * (double)nu_p / (kBoltzmann * T) > 709.782712893384;
+}
+
+// POISON = false leaves e.poison false: for callers that apply the (rare)
+// Theta_e <= 1e-4 case once per sample instead of once per frequency.
+template <bool POISON = true>
 __host__ __device__ inline EmisF emission_nu(const EmisPre &p, float nu_p, float s6, float kf)
 {
     EmisF e;
@@ -873,10 +884,8 @@ __host__ __device__ inline EmisF emission_nu(const EmisPre &p, float nu_p, float
         e.cb = 0.f;
         e.kap = 0.f;
         e.s6 = 0.f;
-        if (!(p.Te > 1e-4f)) {
-            double T = (double)p.Te * kElectronMass * kC * kC / kBoltzmann;
-            e.poison = kPlanck * (double)nu_p / (kBoltzmann * T) > 709.782712893384;
-        }
+        if (POISON && !(p.Te > 1e-4f))
+            e.poison = planck_poison(p.Te, nu_p);
         return e;
     }
     const float cb = s6 * s6;                                         // X^(1/3)
@@ -1124,13 +1133,17 @@ __host__ __device__ inline void gate_share(Accum<NF, GRAD> &acc, int f, float ta
 
 // The thermal fit's emissivity factor of frequency f0 + f at a sample (the
 // part of transfer_sample that the gate-window pass below evaluates again).
+__host__ __device__ inline float plasma_nu(const Params &P, float gdop, int f)
+{
+    const float nu_p = gdop * P.fnu[f];
+    return nu_p <= 0.f ? 1.f : nu_p;
+}
+
 __host__ __device__ inline EmisF thermal_nu(const Params &P, const EmisPre &pre, float gdop,
                                             float lgs, float rg3, int f)
 {
-    float nu_p = gdop * P.fnu[f];
-    if (nu_p <= 0.f)
-        nu_p = 1.f;
-    return emission_nu(pre, nu_p, fexp2(fmaf(lgs, 1.f / 6.f, P.flnu6[f])), rg3 * P.fkap[f]);
+    return emission_nu<false>(pre, plasma_nu(P, gdop, f),
+                              fexp2(fmaf(lgs, 1.f / 6.f, P.flnu6[f])), rg3 * P.fkap[f]);
 }
 
 template <int NF, bool GRAD, int EMIS = kThSynch>
@@ -1188,7 +1201,7 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
         const float g = F.gdop;
         const EmisF e = thermal_nu(P, pre, g, lgs, rg3, f0 + f);
         const float jx = e.jh * fexp(-e.cb);
-        float dtau = e.jh != 0.f ? g * dl * jx * e.kap : (e.poison ? NAN : 0.f);
+        float dtau = e.jh != 0.f ? g * dl * jx * e.kap : 0.f;   // poison: after the loop
         kahan_add(acc.tau[f], acc.ctau[f], dtau);
         const float tau = acc.tau[f];
         const float dI = (float)kE2C * jx * fexp(-tau) * dlg2;
@@ -1220,6 +1233,24 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
                 gate_share(acc, f, tau, dI, wf);
         } else {
             dev = fminf(dev, fabsf(tau - lnth));
+        }
+    }
+    if constexpr (EMIS == kThSynch) {
+        // Theta_e <= 1e-4: j = 0 at every frequency, and where the reference's
+        // FP64 Planck exponent overflows its absorptivity is 0/0, which makes
+        // tau NaN (and with it the rest of the ray's contributions).  The
+        // loop above added dtau = 0 instead (and dI = 0, which changes no
+        // sum); setting the accumulators NaN here gives the same state as
+        // adding the NaN there, without a branch per frequency.
+        if (__builtin_expect(!(pre.Te > 1e-4f), 0)) {
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                if (planck_poison(pre.Te, plasma_nu(P, F.gdop, f0 + f))) {
+                    acc.tau[f] = acc.ctau[f] = NAN;
+                    if (GRAD)
+                        acc.dtau[f][0] = acc.dtau[f][1] = NAN;
+                }
+            }
         }
     }
     if constexpr (EMIS == kThSynch && NF > 1) {

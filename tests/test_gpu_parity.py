@@ -63,6 +63,43 @@ def test_c0_image_matches_oracle(tracer, dump128):
     assert spec[0, 0] == pytest.approx(gpu.sum(), rel=1e-14)
 
 
+@pytest.mark.parametrize("nf", [1, 3, 8])
+def test_cold_cells_poison_rays_as_the_reference(tracer, dump128, nf):
+    """Cells with u < 0 give Theta_e = 1e-20 (raptor_harm_model.c:274-276):
+    j = 0 there, and the reference's FP64 Planck exponent overflows, so its
+    absorptivity is 0/0 and tau NaN, which stops the ray's later
+    contributions (the gate test fails from then on).  A shell of such cells
+    inside the torus (r ~ 6.0-9.2 M, X2 < 3/4): images and tangents at 1,
+    3 and 8 frequencies against the oracle, every pixel; the shell changes
+    tens of pixels by more than 1e-3 of the image's largest (the flux by
+    1-12%), and no output is NaN."""
+    import copy
+    cold = copy.copy(dump128)
+    cold.prims = dump128.prims.copy()
+    cold.prims[1, 56:72, :96, :] = -np.abs(cold.prims[1, 56:72, :96, :])
+    W = 32
+    freqs = [FREQ] if nf == 1 else list(86e9 * (690.0 / 86.0) ** (np.arange(nf) / (nf - 1)))
+    P = _setup(tracer, cold, W)
+    try:
+        img, dimg, _ = tracer.render([INCL], freqs, grad=True)
+    finally:
+        tracer.load_dump(dump128)
+    oimg, odimg = O.render(P, cold.prims, freqs, grad=True)
+    warm = O.render(P, dump128.prims, freqs)
+    assert np.all(np.isfinite(img)) and np.all(np.isfinite(dimg))
+    for f in range(nf):
+        gpu, ora = img[0, :, f], oimg[:, f]
+        r = rel_err(gpu, ora)
+        assert abs(gpu.sum() / ora.sum() - 1) < FLUX_TOL, f
+        assert r.max() < PIX_TOL, (f, r.max())
+        changed = np.abs(ora - warm[:, f]) > 1e-3 * np.abs(warm[:, f]).max()
+        assert changed.sum() >= 20, f
+        ipix = np.maximum(ora, 1e-6 * ora.max())
+        for q, p in enumerate((1.0, R_HIGH)):
+            g, o = dimg[0, :, f, q], odimg[:, f, q]
+            assert (np.abs(g - o) * p / ipix).max() < 1e-3, (f, q)
+
+
 @pytest.mark.parametrize("W,H,incl", [(37, 23, 60.0), (1, 1, 30.0), (16, 16, 1.0),
                                       (16, 16, 179.0), (24, 8, 90.0)])
 def test_ragged_sizes_and_inclinations(tracer, dump128, W, H, incl):
