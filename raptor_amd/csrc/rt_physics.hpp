@@ -879,6 +879,20 @@ __host__ __device__ inline EmisF emission_nu(const EmisPre &p, float nu_p, float
 {
     EmisF e;
     e.poison = false;
+    if constexpr (!POISON) {
+        // branch-free: the fit's factors for every lane, zeros selected
+        // where the gates close (the lanes' values are the same as below;
+        // a gated lane's factors may be inf or NaN and are discarded)
+        const bool zero = nu_p == 1.f || !p.on;
+        const float cb = s6 * s6;
+        const float ff = cb * s6 + 1.8877486253633868f * s6;
+        const float kap = fexpm1(p.xk * nu_p) * kf;
+        e.cb = zero ? 0.f : cb;
+        e.s6 = zero ? 0.f : s6;
+        e.jh = zero ? 0.f : p.jpre * (ff * ff);
+        e.kap = zero ? 0.f : kap;
+        return e;
+    }
     if (nu_p == 1.f || !p.on) {
         e.jh = 0.f;
         e.cb = 0.f;
