@@ -203,9 +203,9 @@ double rt_jansky_factor(const rt_ctx *ctx);
  * run the split launch below in one chunk, the longest rays -- the pixels
  * within 0.3 M of the camera's critical curve (the photon ring) and those
  * whose rays turn in theta within arcsin 0.02 of the spin axis (where the
- * step-size rule shrinks the steps) -- are traced from the camera by a lead
- * kernel on a second stream of the context from the start of the render,
- * beside the approach kernel, instead of after it;
+ * step-size rule shrinks the steps), at most 5% of the image -- are traced
+ * from the camera by a lead kernel on a second stream of the context from
+ * the start of the render, beside the approach kernel, instead of after it;
  * 0 = library default (ring).  approach: the share of the rays, in claim order,
  * whose camera-to-grid leg runs first in a kernel of its own: 1..100
  * percent, RT_APPROACH_NONE for none, 0 = library default (all rays for

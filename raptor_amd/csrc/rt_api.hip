@@ -1068,11 +1068,14 @@ static int ensure_state(rt_ctx *c, long long slots, RayState *out)
 // where the step-size rule (integrator.c:109-122) shrinks the steps with
 // min(X2, 1 - X2) (at 1024^2, a = 0.9375, i = 60: up to 1300 steps inside
 // the grid against a mean of 200, in a column of pixels from the shadow to
-// the image's edge, the last of which the centre-out order claims last).
-// The lead kernel traces them from the camera on a second stream from the
-// start of the render, beside the approach kernel, instead of after it.
+// the image's edge, the last of which the centre-out order claims last),
+// up to kLeadShare of the image in all (a camera near the axis would put
+// nearly every pixel in the band).  The lead kernel traces them from the
+// camera on a second stream from the start of the render, beside the
+// approach kernel, instead of after it.
 static constexpr double kRingWidth = 0.3;   // M: 12304 of 1024^2 pixels at 40 M, a = 0.9375, i = 60
 static constexpr double kPolarSin = 0.02;   // 12022 more pixels there
+static constexpr double kLeadShare = 0.05;  // of the image: the lead set's largest size
 
 static const rt_ctx::OrderTable *find_order(rt_ctx *c, const long long key[kOrderKey])
 {
@@ -1110,32 +1113,15 @@ static void order_key(int W, int H, int T, double sx, double sy, bool ring, doub
 // L = -alpha sin i, Carter Q = beta^2 + cos^2 i (alpha^2 - a^2)), the theta
 // potential Q + a^2 cos^2 - L^2 cot^2 vanishes at u = cos^2 theta_min, the
 // root in [0, 1] of a^2 u^2 + (Q + L^2 - a^2) u - Q = 0 (u = Q / (Q + L^2)
-// for a = 0); rays with Q <= 0 never leave the equatorial band.
+// for a = 0); rays with Q <= 0 never leave the equatorial band.  At most
+// max_lead pixels in all: a camera near the axis sees most rays turn near it
+// (all of them at i <= 1 deg), so beyond that count the band keeps its
+// pixels of smallest sin theta_min (ties by pixel index).
 static void ring_mask(int W, int H, double sx, double sy, double a, double incl,
-                      double width, double polar, std::vector<char> &mask)
+                      double width, double polar, size_t max_lead, std::vector<char> &mask)
 {
     mask.assign((size_t)W * H, 0);
     const double si = std::sin(incl * kPi / 180.), ci = std::cos(incl * kPi / 180.);
-    for (int y = 0; polar > 0. && y < H; y++) {
-        const double be = -0.5 * H * sy + (y + 0.5) * sy;
-        for (int x = 0; x < W; x++) {
-            const double al = -0.5 * W * sx + (x + 0.5) * sx;
-            const double L = -al * si, Q = be * be + ci * ci * (al * al - a * a);
-            if (!(Q > 0.))
-                continue;
-            double u;
-            if (a * a < 1e-12) {
-                u = Q / (Q + L * L);
-            } else {
-                // the root without cancellation: 2Q / (B + sqrt(B^2 + 4 a^2 Q))
-                const double B = Q + L * L - a * a;
-                const double sq = std::sqrt(B * B + 4. * a * a * Q);
-                u = B > 0. ? 2. * Q / (B + sq) : (sq - B) / (2. * a * a);
-            }
-            if (1. - u < polar * polar)
-                mask[(size_t)y * W + x] = 1;
-        }
-    }
     auto mark = [&](double al, double be) {
         const int x0 = std::max(0, (int)std::floor((al - width + 0.5 * W * sx) / sx - 0.5));
         const int x1 = std::min(W - 1, (int)std::ceil((al + width + 0.5 * W * sx) / sx - 0.5));
@@ -1156,22 +1142,112 @@ static void ring_mask(int W, int H, double sx, double sy, double a, double incl,
             const double ph = 2. * kPi * q / n;
             mark(std::sqrt(27.) * std::cos(ph), std::sqrt(27.) * std::sin(ph));
         }
+    } else {
+        const double r1 = 2. * (1. + std::cos(2. / 3. * std::acos(-a)));
+        const double r2 = 2. * (1. + std::cos(2. / 3. * std::acos(a)));
+        const double lo = std::min(r1, r2), hi = std::max(r1, r2);
+        // (alpha, beta^2) of the orbit at radius r
+        auto curve = [&](double r, double &al, double &b2) {
+            const double xi = (r * r * (3. - r) - a * a * (r + 1.)) / (a * (r - 1.));
+            const double eta = r * r * r * (4. * a * a - r * (r - 3.) * (r - 3.)) /
+                               (a * a * (r - 1.) * (r - 1.));
+            al = -xi / si;
+            b2 = eta + a * a * ci * ci - xi * xi * (ci * ci) / (si * si);
+        };
+        auto mark_r = [&](double r) {
+            double al, b2;
+            curve(r, al, b2);
+            if (b2 >= 0.) {
+                mark(al, std::sqrt(b2));
+                mark(al, -std::sqrt(b2));
+            }
+        };
+        // uniform in r, with samples inserted where consecutive points lie
+        // more than width/4 apart (near the axis, i -> 0, the visible arc
+        // maps from a narrow range of r) and the ends of the visible arc
+        // (beta = 0) found by bisection
+        double rp = lo, alp, b2p;
+        curve(rp, alp, b2p);
+        mark_r(rp);
+        for (int q = 1; q <= n; q++) {
+            const double r = lo + (hi - lo) * q / n;
+            double al, b2;
+            curve(r, al, b2);
+            const bool vp = b2p >= 0., v = b2 >= 0.;
+            // both points beyond the same edge of the image: nothing between
+            // them to mark
+            const double ex = 0.5 * W * sx + width, ey = 0.5 * H * sy + width;
+            const bool beyond = (al > ex && alp > ex) || (al < -ex && alp < -ex) ||
+                                (vp && v && b2 > ey * ey && b2p > ey * ey);
+            if (vp && v && !beyond) {
+                const double gap = std::hypot(al - alp, std::sqrt(b2) - std::sqrt(b2p));
+                const int k = std::min(1 << 16, (int)std::ceil(gap / (0.25 * width)));
+                for (int t = 1; t < k; t++)
+                    mark_r(rp + (r - rp) * t / k);
+            } else if (vp != v) {
+                double x0 = rp, x1 = r;   // b2 changes sign between them
+                for (int it = 0; it < 60; it++) {
+                    const double xm = 0.5 * (x0 + x1);
+                    double alm, b2m;
+                    curve(xm, alm, b2m);
+                    if ((b2m >= 0.) == vp)
+                        x0 = xm;
+                    else
+                        x1 = xm;
+                }
+                const double rb = vp ? x0 : x1, rv = vp ? rp : r;   // end, valid sample
+                double alb, b2b, alv, b2v;
+                curve(rb, alb, b2b);
+                curve(rv, alv, b2v);
+                mark(alb, 0.);
+                // beta grows like sqrt(r - rb) from the end: samples at
+                // r = rb + (rv - rb) s^2 are spaced evenly along the arc
+                const double gap = std::hypot(alv - alb, std::sqrt(b2v));
+                const int k = std::min(1 << 16, (int)std::ceil(gap / (0.25 * width)));
+                for (int t = 1; t < k; t++) {
+                    const double sq = (double)t / k;
+                    mark_r(rb + (rv - rb) * sq * sq);
+                }
+            }
+            mark_r(r);
+            rp = r;
+            alp = al;
+            b2p = b2;
+        }
+    }
+    if (!(polar > 0.))
         return;
+    size_t nring = 0;
+    for (char m : mask)
+        nring += m != 0;
+    std::vector<std::pair<double, int>> band;   // (sin^2 theta_min, pixel)
+    for (int y = 0; y < H; y++) {
+        const double be = -0.5 * H * sy + (y + 0.5) * sy;
+        for (int x = 0; x < W; x++) {
+            const double al = -0.5 * W * sx + (x + 0.5) * sx;
+            const double L = -al * si, Q = be * be + ci * ci * (al * al - a * a);
+            if (mask[(size_t)y * W + x] || !(Q > 0.))
+                continue;
+            double u;
+            if (a * a < 1e-12) {
+                u = Q / (Q + L * L);
+            } else {
+                // the root without cancellation: 2Q / (B + sqrt(B^2 + 4 a^2 Q))
+                const double B = Q + L * L - a * a;
+                const double sq = std::sqrt(B * B + 4. * a * a * Q);
+                u = B > 0. ? 2. * Q / (B + sq) : (sq - B) / (2. * a * a);
+            }
+            if (1. - u < polar * polar)
+                band.emplace_back(1. - u, y * W + x);
+        }
     }
-    const double r1 = 2. * (1. + std::cos(2. / 3. * std::acos(-a)));
-    const double r2 = 2. * (1. + std::cos(2. / 3. * std::acos(a)));
-    const double lo = std::min(r1, r2), hi = std::max(r1, r2);
-    for (int q = 0; q <= n; q++) {
-        const double r = lo + (hi - lo) * q / n;
-        const double xi = (r * r * (3. - r) - a * a * (r + 1.)) / (a * (r - 1.));
-        const double eta = r * r * r * (4. * a * a - r * (r - 3.) * (r - 3.)) /
-                           (a * a * (r - 1.) * (r - 1.));
-        const double b2 = eta + a * a * ci * ci - xi * xi * (ci * ci) / (si * si);
-        if (!(b2 >= 0.))
-            continue;
-        mark(-xi / si, std::sqrt(b2));
-        mark(-xi / si, -std::sqrt(b2));
+    const size_t room = max_lead > nring ? max_lead - nring : 0;
+    if (band.size() > room) {
+        std::nth_element(band.begin(), band.begin() + room, band.end());
+        band.resize(room);
     }
+    for (const auto &b : band)
+        mask[(size_t)b.second] = 1;
 }
 
 // ring: incl and a give the camera; the table's first t->nlead entries are
@@ -1213,7 +1289,8 @@ static int ensure_order(rt_ctx *c, int W, int H, int T, double sx, double sy, bo
         order.resize(n);
         c->orders.reserve(c->orders.size() + 1);
         if (ring)
-            ring_mask(W, H, sx, sy, a, incl, kRingWidth, kPolarSin, lead);
+            ring_mask(W, H, sx, sy, a, incl, kRingWidth, kPolarSin,
+                      (size_t)(kLeadShare * (double)n), lead);
     } catch (const std::bad_alloc &) {
         return fail(c, RT_E_NOMEM, "out of host memory for the claim order");
     }
