@@ -335,11 +335,11 @@ __device__ __forceinline__ void trace_rays(const Params &P, const TraceArgs &A)
                 if constexpr (kInline && kOverlap)
                     cont = ray_step_rk2_overlap_mixed<NF, GRAD, DIM3>(P, A.prims, R, A0, xm);
                 else if constexpr (kInline)
-                    cont = ray_step_mixed<NF, GRAD, DIM3, INT>(P, A.prims, R, xm);
+                    cont = ray_step_mixed<NF, GRAD, DIM3, INT, !RESUME>(P, A.prims, R, xm);
                 else if constexpr (kOverlap)
                     cont = ray_step_rk2_overlap<NF, GRAD, DIM3, EMIS>(P, A.prims, R, A0);
                 else
-                    cont = ray_step<NF, GRAD, DIM3, INT, EMIS>(P, A.prims, R);
+                    cont = ray_step<NF, GRAD, DIM3, INT, EMIS, false, !RESUME>(P, A.prims, R);
             }
             if (!cont) {
                 if (kInline && !xm && needs_exact<EMIS>(R)) {

@@ -320,12 +320,18 @@ __host__ __device__ inline void sincos_small(double x, double &s, double &c)
     c = 1. - x2 * pc;
 }
 
+// UG (the kernels that trace rays from the camera): the series are
+// evaluated on every lane and the full evaluation runs in a wave-uniform
+// branch, taken only when some lane is beyond a guard (approach kernel -5%,
+// the lead kernel's long rays -9%; the resuming kernels run slower so, and
+// keep the per-lane early return).
+template <bool UG = false>
 __host__ __device__ inline Point shifted_point(const Point &p0, double X1m, double X2m,
                                                double e1, double e2, double h)
 {
     const double kh = (1. - h) * (0.5 / kPi);
     const double a2 = 2. * kPi * e2;
-    if (!(fabs(e1) <= 0.031 && fabs(a2) <= 0.095))
+    if (!UG && !(fabs(e1) <= 0.031 && fabs(a2) <= 0.095))
         return make_point(X1m, X2m, h);
     Point p;
     p.r = p0.r * exp_small(e1);
@@ -334,11 +340,19 @@ __host__ __device__ inline Point shifted_point(const Point &p0, double X1m, doub
     p.s2 = p0.s2 * dc + p0.c2 * ds;
     p.c2 = p0.c2 * dc - p0.s2 * ds;
     const double at = kPi * (e2 + kh * (p.s2 - p0.s2));   // theta_m - theta_0
-    if (!(fabs(at) <= 0.095))
+    if (!UG && !(fabs(at) <= 0.095))
         return make_point(X1m, X2m, h);
     sincos_small(at, ds, dc);
     p.s = p0.s * dc + p0.c * ds;
     p.c = p0.c * dc - p0.s * ds;
+    if (UG) {
+        const bool ok = fabs(e1) <= 0.031 && fabs(a2) <= 0.095 && fabs(at) <= 0.095;
+        if (__builtin_expect(wave_any(!ok), 0)) {
+            const Point q = make_point(X1m, X2m, h);
+            if (!ok)
+                p = q;
+        }
+    }
     return p;
 }
 
@@ -460,6 +474,7 @@ __host__ __device__ inline double stepsize(const double y[8], double eps)
 // y's point on entry and of the new point on exit.  The time coordinate
 // X^0 is not advanced: nothing on the path reads it (the metric is
 // stationary; the reference only prints it in its debug geodesic dump).
+template <bool UG = false>
 __host__ __device__ inline void rk2_step_from(double y[8], Point &pt, double dl, double a,
                                               double h, const double A0[4])
 {
@@ -470,24 +485,25 @@ __host__ __device__ inline void rk2_step_from(double y[8], Point &pt, double dl,
         ym[i] = y[i] + hdl * y[4 + i];
     for (int i = 0; i < 4; i++)
         ym[4 + i] = y[4 + i] + hdl * A0[i];
-    Point pm = shifted_point(pt, ym[1], ym[2], ym[1] - y[1], ym[2] - y[2], h);
+    Point pm = shifted_point<UG>(pt, ym[1], ym[2], ym[1] - y[1], ym[2] - y[2], h);
     geodesic_accel(pm, a, h, ym + 4, A);
     const double X1o = y[1], X2o = y[2];
     for (int i = 1; i < 4; i++)
         y[i] = y[i] + dl * ym[4 + i];
     for (int i = 0; i < 4; i++)
         y[4 + i] = y[4 + i] + dl * A[i];
-    pt = shifted_point(pt, y[1], y[2], y[1] - X1o, y[2] - X2o, h);
+    pt = shifted_point<UG>(pt, y[1], y[2], y[1] - X1o, y[2] - X2o, h);
 }
 
 // rk2_step_from with the start point's acceleration A0 given: the same
 // arithmetic, so a caller that has A0 already (computed beside the previous
 // step's transfer sample) gets the same bits.
+template <bool UG = false>
 __host__ __device__ inline void rk2_step(double y[8], Point &pt, double dl, double a, double h)
 {
     double A0[4];
     geodesic_accel(pt, a, h, y + 4, A0);
-    rk2_step_from(y, pt, dl, a, h, A0);
+    rk2_step_from<UG>(y, pt, dl, a, h, A0);
 }
 
 // The steppers of integrator.c:258-263, chosen by int_method
@@ -551,7 +567,7 @@ __host__ __device__ inline void verlet_step(double y[8], Point &pt, double dl, d
     pt = pn;
 }
 
-template <int INT>
+template <int INT, bool UG = false>
 __host__ __device__ inline void geodesic_step(double y[8], Point &pt, double dl, double a,
                                               double h)
 {
@@ -560,7 +576,7 @@ __host__ __device__ inline void geodesic_step(double y[8], Point &pt, double dl,
     else if (INT == kVerlet)
         verlet_step(y, pt, dl, a, h);
     else
-        rk2_step(y, pt, dl, a, h);
+        rk2_step<UG>(y, pt, dl, a, h);
 }
 
 // Pixel part of initialize_photon (metric.c:650-727): k_mu from impact
@@ -1535,7 +1551,7 @@ __host__ __device__ inline bool ray_alive(const Params &P, const Ray<NF, GRAD> &
 // chosen integrator and the escape test.  Returns false when the ray escaped
 // through the outer grid edge (the `break` at integrator.c:271-272); `dl` is
 // the step taken, which the sample of the new point uses.
-template <int INT, int NF, bool GRAD>
+template <int INT, bool UG = false, int NF, bool GRAD>
 __host__ __device__ inline bool ray_advance(const Params &P, Ray<NF, GRAD> &R, double &dl)
 {
     // integrator.c:243-247: a sign change of k^theta is a turning point
@@ -1546,7 +1562,7 @@ __host__ __device__ inline bool ray_advance(const Params &P, Ray<NF, GRAD> &R, d
     if ((R.beta_sign < 0 && R.turns > kMaxOrder) || (R.beta_sign > 0 && R.turns > kMaxOrder + 1))
         R.terminate = true;
     dl = stepsize(R.y, P.stepsize);
-    geodesic_step<INT>(R.y, R.pt, dl, P.a, P.hslope);
+    geodesic_step<INT, UG>(R.y, R.pt, dl, P.a, P.hslope);
     return !(R.y[1] > P.stopx1 && R.y[5] < 0.);
 }
 
@@ -1584,12 +1600,13 @@ __host__ __device__ inline void ray_sample(const Params &P, const float4 *__rest
 
 // One iteration of integrate_geodesic's loop (integrator.c:243-284).
 // Returns false when the ray escaped through the outer grid edge.
-template <int NF, bool GRAD, bool DIM3, int INT = kRK2, int EMIS = kThSynch, bool EXACT = false>
+template <int NF, bool GRAD, bool DIM3, int INT = kRK2, int EMIS = kThSynch, bool EXACT = false,
+          bool UG = false>
 __host__ __device__ inline bool ray_step(const Params &P, const float4 *__restrict__ prims,
                                          Ray<NF, GRAD> &R)
 {
     double dl;
-    if (!ray_advance<INT>(P, R, dl))
+    if (!ray_advance<INT, UG>(P, R, dl))
         return false;
     ray_sample<NF, GRAD, DIM3, EMIS, EXACT>(P, prims, R, dl);
     return true;
@@ -1659,7 +1676,7 @@ __host__ __device__ inline bool approach_ray(const Params &P, int cam, int pix, 
             if (saved)
                 save(R);
             double dl;
-            if (!ray_advance<INT>(P, R, dl))
+            if (!ray_advance<INT, true>(P, R, dl))
                 break;
             if (in_grid(P, R.y)) {
                 if (saved)
@@ -1728,12 +1745,12 @@ __host__ __device__ inline bool ray_step_rk2_overlap(const Params &P,
 // registers: the register allocator then puts its spills in that branch
 // (the 3D image kernel at 1024^2: 8.5 -> 7.7 ms).  Same arithmetic as
 // before, the same bits.
-template <int NF, bool GRAD, bool DIM3, int INT>
+template <int NF, bool GRAD, bool DIM3, int INT, bool UG = false>
 __host__ __device__ inline bool ray_step_mixed(const Params &P, const float4 *__restrict__ prims,
                                                Ray<NF, GRAD> &R, bool exact)
 {
     double dl;
-    if (!ray_advance<INT>(P, R, dl))
+    if (!ray_advance<INT, UG>(P, R, dl))
         return false;
     if (in_grid(P, R.y)) {
         if (__builtin_expect(exact, 0)) {
