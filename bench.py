@@ -571,9 +571,9 @@ def main():
                     help="the headline workload only")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(pixels shard trivially; see the sharding design). Use --gpus 1.",
-              file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(pixels, cameras and frequencies shard without communication). "
+              "Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "c2":

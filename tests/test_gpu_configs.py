@@ -8,7 +8,7 @@ against the FP64 oracle on pixel samples:
       512^2, R_high in {1, 10, 40, 160}, against central differences (step
       1e-3) of the oracle
 
-Bar (north star): flux within 1e-4 relative of the oracle, every pixel within
+Bar: flux within 1e-4 relative of the oracle, every pixel within
 1e-3 relative with an absolute floor of 1e-6 max(I); gradients as in
 test_gpu_grad.py.  The flux of a sampled image is the sum over its sampled
 pixels."""

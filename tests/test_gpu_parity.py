@@ -1,6 +1,6 @@
 """Parity of the HIP path (through the C-ABI) with the FP64 CPU oracle.
 
-Bar (north star): total flux within 1e-4 relative, every pixel within 1e-3
+Bar: total flux within 1e-4 relative, every pixel within 1e-3
 relative with an absolute floor of 1e-6 * max(I); building blocks to
 near-FP64 rounding.
 """
