@@ -1123,6 +1123,8 @@ static void ring_mask(int W, int H, double sx, double sy, double a, double incl,
     mask.assign((size_t)W * H, 0);
     const double si = std::sin(incl * kPi / 180.), ci = std::cos(incl * kPi / 180.);
     auto mark = [&](double al, double be) {
+        if (!(std::fabs(al) <= 1e9 && std::fabs(be) <= 1e9))   // off the image, or NaN
+            return;
         const int x0 = std::max(0, (int)std::floor((al - width + 0.5 * W * sx) / sx - 0.5));
         const int x1 = std::min(W - 1, (int)std::ceil((al + width + 0.5 * W * sx) / sx - 0.5));
         const int y0 = std::max(0, (int)std::floor((be - width + 0.5 * H * sy) / sy - 0.5));
@@ -1181,7 +1183,8 @@ static void ring_mask(int W, int H, double sx, double sy, double a, double incl,
                                 (vp && v && b2 > ey * ey && b2p > ey * ey);
             if (vp && v && !beyond) {
                 const double gap = std::hypot(al - alp, std::sqrt(b2) - std::sqrt(b2p));
-                const int k = std::min(1 << 16, (int)std::ceil(gap / (0.25 * width)));
+                const double kk = std::ceil(gap / (0.25 * width));
+                const int k = kk < 65536. ? (int)kk : 65536;   // NaN and inf too
                 for (int t = 1; t < k; t++)
                     mark_r(rp + (r - rp) * t / k);
             } else if (vp != v) {
@@ -1203,7 +1206,8 @@ static void ring_mask(int W, int H, double sx, double sy, double a, double incl,
                 // beta grows like sqrt(r - rb) from the end: samples at
                 // r = rb + (rv - rb) s^2 are spaced evenly along the arc
                 const double gap = std::hypot(alv - alb, std::sqrt(b2v));
-                const int k = std::min(1 << 16, (int)std::ceil(gap / (0.25 * width)));
+                const double kk = std::ceil(gap / (0.25 * width));
+                const int k = kk < 65536. ? (int)kk : 65536;
                 for (int t = 1; t < k; t++) {
                     const double sq = (double)t / k;
                     mark_r(rb + (rv - rb) * sq * sq);
