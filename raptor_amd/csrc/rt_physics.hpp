@@ -1214,7 +1214,7 @@ __host__ __device__ inline void transfer_sample(const Params &P, const Fluid &F,
         if constexpr (EMIS != kThSynch) {
             static_assert(!GRAD, "the tangents are those of the thermal fit");
             if (f >= P.nf)
-                break;
+                continue;   // (not break: the loop stays fully unrolled)
             float g = F.gdop;
             float nu_p = g * P.fnu[f0 + f];
             if (nu_p <= 0.f) {
