@@ -273,6 +273,29 @@ def test_ring_order_lead_launch_is_bit_identical(tracer):
             tracer.free(p)
 
 
+def test_ring_order_lead_launch_non_square_image(tracer):
+    """The lead set of a 2048 x 512 image (2^20 rays, the split launch; the
+    ring and polar band in camera units of non-square pixel counts) leaves
+    the image bit-identical to the centre order's, at 60 and 150 deg."""
+    W, H = 2048, 512
+    _params(tracer, W, H)
+    n = W * H
+    d = tracer.alloc(n * 8)
+    try:
+        for incl in (INCL, 150.0):
+            out = {}
+            for order in (rt.ORDER_CENTRE, rt.ORDER_RING):
+                tracer.set_launch_config(order=order)
+                tracer.render_device([incl], [FREQ], d)
+                tracer.synchronize()
+                out[order] = (tracer.d2h(d, (n,), np.float64), tracer.last_kernel_times()["lead"])
+            assert np.array_equal(out[rt.ORDER_CENTRE][0], out[rt.ORDER_RING][0]), incl
+            assert out[rt.ORDER_RING][1] > 0.0 and tracer.claim_order(incl)[1] > 1000
+    finally:
+        tracer.set_launch_config()
+        tracer.free(d)
+
+
 def _critical_curve(a, incl_deg, n=200001, ds=0.002):
     """Bardeen's critical curve (impact parameters alpha, beta of the
     spherical photon orbits) for a camera at incl_deg around spin a, with
