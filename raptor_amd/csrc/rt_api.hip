@@ -221,13 +221,11 @@ __device__ inline void load_state(const RayState &S, int i, Ray<NF, GRAD> &R)
 // the grid (approach_ray) and leaves it saved for the trace kernel.  Every
 // ray's approach is about ln(r_cam / r_out) / STEPSIZE steps with no fluid
 // work, so these waves run without the idle lanes they would have beside rays
-// inside the grid.  RK2 and Verlet at 4 waves per SIMD (128 VGPRs, 36 of
-// the RK2 step's 157 spilled): beside a lead wave (232 VGPRs) two approach
-// waves fit instead of one, and the kernel runs ~3% faster than at 3 waves.
-// RK4's four accelerations per step would spill 40 VGPRs even at 3 waves
-// (six reloads a step), and at 2 waves (205 VGPRs) it runs 2% faster.
+// inside the grid.  3 waves per SIMD; RK4's four accelerations per step
+// would spill 40 VGPRs there (six reloads a step), and at 2 waves (205
+// VGPRs) it runs 2% faster.
 template <int INT>
-__global__ __launch_bounds__(64, INT == kRK4 ? 2 : 4) void approach_kernel(const Params P,
+__global__ __launch_bounds__(64, INT == kRK4 ? 2 : 3) void approach_kernel(const Params P,
                                                                           const TraceArgs A)
 {
     const int i = blockIdx.x * 64 + threadIdx.x;
