@@ -359,11 +359,16 @@ __device__ __forceinline__ void trace_rays(const Params &P, const TraceArgs &A)
         // A wave holding a ray that has run kLongRay steps (a ray near the
         // photon ring: these orbit the hole and take up to a few thousand
         // steps) issues ahead of the other waves on its SIMD, so that the
-        // launch does not end waiting on it.
-        if (__ballot(active && R.steps > kLongRay))
-            __builtin_amdgcn_s_setprio(2);
-        else
-            __builtin_amdgcn_s_setprio(0);
+        // launch does not end waiting on it.  Kernels tracing from the
+        // camera only: the resuming kernels' longest rays are the lead
+        // kernel's, and there the two ballots a step cost more (1024^2
+        // render -2%).
+        if constexpr (!RESUME) {
+            if (__ballot(active && R.steps > kLongRay))
+                __builtin_amdgcn_s_setprio(2);
+            else
+                __builtin_amdgcn_s_setprio(0);
+        }
         const unsigned long long act = __ballot(active);
         exhausted = __ballot(exhausted) != 0ull;
         const int idle = 64 - __popcll(act);
