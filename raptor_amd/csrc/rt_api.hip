@@ -443,9 +443,9 @@ __global__ __launch_bounds__(64) void exact_kernel(const Params P, const TraceAr
         // is the decision: only the FP64 transfer is traced again
         const bool alone = (e.y & (e.y - 1)) == 0;
         if (alone)
-            exact_retrace<1, GRAD, DIM3, INT, false>(P, A.prims, R, x, wf, f);
+            exact_retrace<1, GRAD, DIM3, INT, false, RESUME && INT == kRK2>(P, A.prims, R, x, wf, f);
         else
-            exact_retrace<1, GRAD, DIM3, INT>(P, A.prims, R, x, wf, f);
+            exact_retrace<1, GRAD, DIM3, INT, true, RESUME && INT == kRK2>(P, A.prims, R, x, wf, f);
         if (alone || wf[0] > kGateFlip) {
             const size_t o = (size_t)ray * P.nf + f;
             A.image[o] = (double)x.I[0];

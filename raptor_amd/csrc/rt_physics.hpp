@@ -1629,7 +1629,7 @@ __host__ __device__ inline bool ray_step(const Params &P, const float4 *__restri
 // WF = false: the decision is known already (the first trace's window
 // share of this frequency alone, when it was the only one in the window),
 // so only the FP64 transfer is traced, to its own crossing.
-template <int NF, bool GRAD, bool DIM3, int INT, bool WF = true>
+template <int NF, bool GRAD, bool DIM3, int INT, bool WF = true, bool OVL = false>
 __host__ __device__ inline void exact_retrace(const Params &P, const float4 *__restrict__ prims,
                                               Ray<NF, GRAD> &R, Accum<NF, GRAD> &x,
                                               float (&wf)[NF], int f0 = 0)
@@ -1638,6 +1638,51 @@ __host__ __device__ inline void exact_retrace(const Params &P, const float4 *__r
 #pragma unroll
     for (int f = 0; f < NF; f++)
         wf[f] = 0.f;
+    if constexpr (OVL) {
+        // RK2 rays inside the grid (exact_kernel's resumed rays, a lane
+        // each): the new point's samples are evaluated whether or not it is
+        // inside (kept only if it is) beside the next step's first
+        // acceleration, so that the two FP64 chains interleave in one lane;
+        // the same arithmetic as below
+        static_assert(INT == kRK2, "the overlap is RK2's");
+        double A0[4];
+        geodesic_accel(R.pt, P.a, P.hslope, R.y + 4, A0);
+        while (ray_alive<NF, GRAD>(P, R)) {
+            double dl;
+            if (!ray_advance_from(P, R, dl, A0))
+                break;
+            const bool inside = in_grid(P, R.y);
+            Accum<NF, GRAD> xt = x;
+            transfer_sample_f64<NF, GRAD, DIM3>(P, prims, R.pt, R.y, fabs(dl), xt, f0);
+            [[maybe_unused]] bool s32 = false;
+            [[maybe_unused]] Accum<NF, GRAD> at = R.acc;
+            [[maybe_unused]] float wt[NF];
+            if constexpr (WF) {
+                Fluid F;
+                s32 = fluid_sample<DIM3, true>(P, prims, R.pt, R.y, F);
+#pragma unroll
+                for (int f = 0; f < NF; f++)
+                    wt[f] = wf[f];
+                transfer_sample<NF, GRAD, kThSynch>(P, F, fabs(dl), at, wt, f0);
+            }
+            geodesic_accel(R.pt, P.a, P.hslope, R.y + 4, A0);
+            if (inside) {
+                if constexpr (WF) {
+                    if (s32) {
+                        R.samples++;
+                        R.acc = at;
+#pragma unroll
+                        for (int f = 0; f < NF; f++)
+                            wf[f] = wt[f];
+                    }
+                }
+                x = xt;
+                R.terminate = R.terminate || ((!WF || opaque(P, R.acc)) && opaque_f64(P, x));
+            }
+            R.steps++;
+        }
+        return;
+    }
     while (ray_alive<NF, GRAD>(P, R)) {
         double dl;
         if (!ray_advance<INT>(P, R, dl))
