@@ -418,15 +418,150 @@ __global__ __launch_bounds__(64, 3) void trace_kernel_3w(const Params P, const T
 // frequency.  A kernel of its own: its FP64 arithmetic needs more registers
 // than the multi-frequency trace kernels can give.  Consecutive entries go
 // to different waves, so that those few rays run on different SIMDs.
+// Lane k's value of v (k the same on every lane).
+__device__ inline double lane_read(double v, int k)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, k);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), k);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// exact_retrace<1, GRAD, DIM3, INT, false> (the FP64 transfer alone, one
+// frequency) by a whole wave: every lane integrates the same geodesic, 64
+// steps at a time, lane k keeping the point of the batch's step k; the lanes
+// then evaluate their points' FP64 samples at once (sample_terms_f64), and
+// the optical depth, intensity and tangents are summed over the samples in
+// step order as the one-lane re-trace sums them, up to the sample after
+// which the frequency is opaque.  The same arithmetic, so the same bits; the
+// batch's geodesic steps after that sample are dropped.  One lane's re-trace
+// costs a wave's issue per step either way; here the FP64 sample, most of a
+// step, is shared by the batch.
+template <bool GRAD, bool DIM3, int INT>
+__device__ void exact_retrace_wave(const Params &P, const float4 *__restrict__ prims,
+                                   Ray<1, GRAD> &R, Accum<1, GRAD> &x, int f)
+{
+    const int lane = (int)threadIdx.x;
+    const double lnth = 6.907755278982137;   // log(1000.)
+    x.clear();
+    bool done = false;
+    while (!done) {
+        double my[8];
+        Point mpt = R.pt;
+        double mdl = 0.;
+        bool min = false;
+        int nb = 0;
+#pragma unroll 1
+        for (int k = 0; k < 64; k++) {
+            double dl;
+            if (!ray_alive<1, GRAD>(P, R) || !ray_advance<INT>(P, R, dl)) {
+                done = true;
+                break;
+            }
+            if (lane == k) {
+#pragma unroll
+                for (int q = 0; q < 8; q++)
+                    my[q] = R.y[q];
+                mpt = R.pt;
+                mdl = dl;
+                min = in_grid(P, R.y);
+            }
+            R.steps++;
+            nb++;
+        }
+        SampleF64 T = {};
+        bool ok = false;
+        if (lane < nb && min) {
+            Fluid64 F;
+            if (fluid_sample_f64<DIM3>(P, prims, mpt, my, F)) {
+                ok = true;
+                T = sample_terms_f64<GRAD>(F, synch_f64(F), P.freq[f], fabs(mdl) * P.Rg);
+            }
+        }
+        // optical depth after each sample, in step order, to the sample that
+        // makes the frequency opaque (the one-lane re-trace stops there)
+        const unsigned long long okm = __ballot(ok);
+        double tau = tau_f64(x.tau[0], x.ctau[0]), mtau = 0.;
+        int last = 64;
+        for (unsigned long long m = okm; m; m &= m - 1) {
+            const int k = __builtin_ctzll(m);
+            tau = tau + lane_read(T.dtau, k);
+            if (lane == k)
+                mtau = tau;
+            if (!(tau < lnth)) {
+                last = k;
+                break;
+            }
+        }
+        const double metau = dexp(-mtau);
+        for (unsigned long long m = okm; m; m &= m - 1) {
+            const int k = __builtin_ctzll(m);
+            if (k > last)
+                break;
+            SampleF64 t;
+            t.dtau = lane_read(T.dtau, k);
+            t.j = lane_read(T.j, k);
+            t.dl = lane_read(T.dl, k);
+            t.gi2 = lane_read(T.gi2, k);
+            if (GRAD) {
+                t.dlnj0 = lane_read(T.dlnj0, k);
+                t.dlnj1 = lane_read(T.dlnj1, k);
+                t.dlna1 = lane_read(T.dlna1, k);
+            }
+            const double tk = lane_read(mtau, k);
+            set_tau_f64(x.tau[0], x.ctau[0], tk);
+            add_sample_f64(x, 0, t, tk, lane_read(metau, k));
+        }
+        if (last < 64)
+            done = true;
+    }
+}
+
 template <bool GRAD, bool DIM3, int INT, bool RESUME>
 __global__ __launch_bounds__(64) void exact_kernel(const Params P, const TraceArgs A)
 {
     const unsigned n = *A.xcount;
+    if ((int)blockIdx.y >= P.nf) {
+        // the rows past the frequencies: a wave per listed ray with one
+        // frequency in its gate mask (exact_retrace_wave)
+        const unsigned rows = gridDim.y - (unsigned)P.nf;
+        const unsigned stride = gridDim.x * rows;
+        for (unsigned i = blockIdx.x + gridDim.x * (blockIdx.y - (unsigned)P.nf); i < n;
+             i += stride) {
+            const int2 e = A.xlist[i];
+            const unsigned mask = (unsigned)e.y;
+            if (mask == 0u || (mask & (mask - 1u)) != 0u)
+                continue;
+            const int f = __builtin_ctz(mask);
+            const int q = e.x;
+            int ray;
+            if (RESUME && q < A.nsaved)
+                ray = state_ray(A.st, q);
+            else if (!work_to_ray(P, A, A.w0 + q, ray))
+                continue;
+            Ray<1, GRAD> R;
+            start_ray<RESUME>(P, A, ray, q, R);
+            Accum<1, GRAD> x;
+            exact_retrace_wave<GRAD, DIM3, INT>(P, A.prims, R, x, f);
+            if (threadIdx.x == 0) {
+                const size_t o = (size_t)ray * P.nf + f;
+                A.image[o] = (double)x.I[0];
+                if (GRAD) {
+                    A.dimage[o * 2 + 0] = x.dI[0][0];
+                    A.dimage[o * 2 + 1] = x.dI[0][1];
+                }
+            }
+        }
+        return;
+    }
     const int f = blockIdx.y;
     const unsigned stride = gridDim.x * 64u;
     for (unsigned i = blockIdx.x + gridDim.x * threadIdx.x; i < n; i += stride) {
         const int2 e = A.xlist[i];
-        if (!((e.y >> f) & 1))
+        // a ray listed with one frequency in the gate window was listed for
+        // that frequency's own share (the first trace summed no other), which
+        // is the decision: the wave rows trace its FP64 transfer again
+        if (!((e.y >> f) & 1) || (e.y & (e.y - 1)) == 0)
             continue;
         const int q = e.x;
         int ray;
@@ -438,15 +573,8 @@ __global__ __launch_bounds__(64) void exact_kernel(const Params P, const TraceAr
         start_ray<RESUME>(P, A, ray, q, R);
         Accum<1, GRAD> x;
         float wf[1];
-        // a ray listed with one frequency in the gate window was listed for
-        // that frequency's own share (the first trace summed no other), which
-        // is the decision: only the FP64 transfer is traced again
-        const bool alone = (e.y & (e.y - 1)) == 0;
-        if (alone)
-            exact_retrace<1, GRAD, DIM3, INT, false, RESUME && INT == kRK2>(P, A.prims, R, x, wf, f);
-        else
-            exact_retrace<1, GRAD, DIM3, INT, true, RESUME && INT == kRK2>(P, A.prims, R, x, wf, f);
-        if (alone || wf[0] > kGateFlip) {
+        exact_retrace<1, GRAD, DIM3, INT, true, RESUME && INT == kRK2>(P, A.prims, R, x, wf, f);
+        if (wf[0] > kGateFlip) {
             const size_t o = (size_t)ray * P.nf + f;
             A.image[o] = (double)x.I[0];
             if (GRAD) {
@@ -932,7 +1060,11 @@ static hipError_t dispatch_exact(rt_ctx *c, const Params &P, const TraceArgs &A)
         return hipSuccess;
     const long long gx = std::max(1LL, std::min<long long>(4LL * c->num_cu,
                                                            ((long long)A.total + 63) / 64));
-    hipLaunchKernelGGL((exact_kernel<GRAD, DIM3, INT, RESUME>), dim3((unsigned)gx, (unsigned)P.nf),
+    // and rows of waves for the rays with one frequency in the window, about
+    // 8 waves per CU
+    const long long wrows = std::max(1LL, (8LL * c->num_cu + gx - 1) / gx);
+    hipLaunchKernelGGL((exact_kernel<GRAD, DIM3, INT, RESUME>),
+                       dim3((unsigned)gx, (unsigned)(P.nf + wrows)),
                        dim3(64), 0, c->stream, P, A);
     return hipGetLastError();
 }

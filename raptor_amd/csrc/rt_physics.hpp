@@ -1383,6 +1383,95 @@ __host__ __device__ inline bool fluid_sample_f64(const Params &P, const float4 *
 // accumulator's (tau, ctau) pair; I and, with GRAD, the tangents accumulate
 // as in transfer_sample (accumulator f: the render's frequency f0 + f).
 // Returns false where the sigma cut skips the sample.
+// One frequency's terms of an FP64 sample: its optical depth dtau and the
+// factors of its intensity dI = j exp(-tau) dl gi2 (tau after the sample),
+// with the logarithmic derivatives the tangents need.
+struct SampleF64 {
+    double dtau, j, dl, gi2;
+    double dlnj0, dlnj1, dlna1;
+};
+
+// The per-sample quantities of the thermal fit in FP64 (set once a sample).
+struct SynchF64 {
+    double nu_s, xk;
+    bool on;
+};
+
+__host__ __device__ inline SynchF64 synch_f64(const Fluid64 &F)
+{
+    SynchF64 S;
+    const double sth = dsqrt(1. - F.mu * F.mu);
+    const double nu_c = kElectronCharge * F.B / (2. * kPi * kElectronMass * kC);
+    S.nu_s = (2. / 9.) * nu_c * F.thetae * F.thetae * sth;
+    S.on = !(fabs(sth) < 1.e-10) && F.thetae > 1e-4;
+    S.xk = kPlanck / (kElectronMass * kC * kC) * drcp(F.thetae);   // x = h nu_p / k T_e
+    return S;
+}
+
+template <bool GRAD>
+__host__ __device__ inline SampleF64 sample_terms_f64(const Fluid64 &F, const SynchF64 &S,
+                                                      double nu, double dl)
+{
+    const double c1112 = 1.8877486253633868;   // 2^(11/12)
+    SampleF64 T;
+    double nu_p = F.gdop * nu;
+    if (nu_p <= 0.)
+        nu_p = 1.;
+    const double x = S.xk * nu_p;
+    double j = 0., s6 = 0.;
+    if (S.on && nu_p != 1.) {
+        const double X = nu_p * drcp(S.nu_s);
+        s6 = dexp(dlog(X) * (1. / 6.));        // X^(1/6)
+        const double ff = s6 * s6 * s6 + c1112 * s6;
+        j = F.ne * (1.4142135623730951 * kPi / 3.) * S.nu_s * drcp(2. * F.thetae * F.thetae) *
+            (ff * ff) * dexp(-(s6 * s6)) * kE2C;
+    }
+    // alpha = j (e^2/ch) / B_nu / (e^2/c) with B_nu = 2 nu^3 / c^2 / (e^x - 1),
+    // written as a product: where e^x - 1 is 0 (B_nu infinite) it is 0, and
+    // where e^x overflows (B_nu 0) it is infinite, or NaN for j = 0 -- the
+    // reference's divisions give the same
+    const double alpha = j * (kE2CH / kE2C) * (dexp(x) - 1.) * (0.5 * kC * kC) *
+                         drcp(nu_p * nu_p * nu_p);
+    T.dtau = nu_p * drcp(nu) * alpha * dl;
+    const double gi = nu * drcp(nu_p);
+    T.j = j;
+    T.dl = dl;
+    T.gi2 = gi * gi;
+    T.dlnj0 = T.dlnj1 = T.dlna1 = 0.;
+    if (GRAD && j != 0.) {
+        const double cb = s6 * s6, sx = cb * s6;
+        const double Dx = 2. * (0.5 * sx + c1112 / 6. * s6) * drcp(sx + c1112 * s6) - cb / 3.;
+        const double dlnB = 0.5 * (F.B - 1e-20) * drcp(F.B);
+        const double dlnT = F.dthetae * drcp(F.thetae);
+        const double dlnBnu = x * drcp(-dexpm1(-x));
+        T.dlnj0 = 1. + dlnB * (1. - Dx);
+        T.dlnj1 = -2. * Dx * dlnT;
+        T.dlna1 = T.dlnj1 - dlnBnu * dlnT;
+    }
+    return T;
+}
+
+// A sample's terms into frequency f of the accumulator, its optical depth
+// already at tau (after the sample; etau = exp(-tau)).
+template <int NF, bool GRAD>
+__host__ __device__ inline void add_sample_f64(Accum<NF, GRAD> &acc, int f, const SampleF64 &T,
+                                               double tau, double etau)
+{
+    const double lnth = 6.907755278982137;   // log(1000.)
+    const double dI = T.j * etau * T.dl * T.gi2;
+    if (GRAD && T.dtau != 0.) {
+        acc.dtau[f][0] += T.dtau * T.dlnj0;
+        acc.dtau[f][1] += T.dtau * T.dlna1;
+    }
+    if (tau < lnth && !(tau < 0.)) {
+        acc.I[f] += (float)dI;
+        if (GRAD && dI != 0.) {
+            acc.dI[f][0] += dI * (T.dlnj0 - acc.dtau[f][0]);
+            acc.dI[f][1] += dI * (T.dlnj1 - acc.dtau[f][1]);
+        }
+    }
+}
+
 template <int NF, bool GRAD, bool DIM3>
 __host__ __device__ inline bool transfer_sample_f64(const Params &P,
                                                     const float4 *__restrict__ prims,
@@ -1393,66 +1482,18 @@ __host__ __device__ inline bool transfer_sample_f64(const Params &P,
     Fluid64 F;
     if (!fluid_sample_f64<DIM3>(P, prims, pt, y, F))
         return false;
-    const double lnth = 6.907755278982137;   // log(1000.)
     const double dl = dlabs * P.Rg;
-    const double sth = dsqrt(1. - F.mu * F.mu);
-    const double nu_c = kElectronCharge * F.B / (2. * kPi * kElectronMass * kC);
-    const double nu_s = (2. / 9.) * nu_c * F.thetae * F.thetae * sth;
-    const bool on = !(fabs(sth) < 1.e-10) && F.thetae > 1e-4;
-    const double xk = kPlanck / (kElectronMass * kC * kC) * drcp(F.thetae);   // x = h nu_p / k T_e
-    const double c1112 = 1.8877486253633868;                            // 2^(11/12)
+    const SynchF64 S = synch_f64(F);
 #pragma unroll
     for (int f = 0; f < NF; f++) {
         // unrolled (the accumulators stay in registers); frequencies past
         // P.nf are skipped
         if (f0 + f >= P.nf)
             continue;
-        const double nu = P.freq[f0 + f];
-        double nu_p = F.gdop * nu;
-        if (nu_p <= 0.)
-            nu_p = 1.;
-        const double x = xk * nu_p;
-        double j = 0., s6 = 0.;
-        if (on && nu_p != 1.) {
-            const double X = nu_p * drcp(nu_s);
-            s6 = dexp(dlog(X) * (1. / 6.));        // X^(1/6)
-            const double ff = s6 * s6 * s6 + c1112 * s6;
-            j = F.ne * (1.4142135623730951 * kPi / 3.) * nu_s * drcp(2. * F.thetae * F.thetae) *
-                (ff * ff) * dexp(-(s6 * s6)) * kE2C;
-        }
-        // alpha = j (e^2/ch) / B_nu / (e^2/c) with B_nu = 2 nu^3 / c^2 / (e^x - 1),
-        // written as a product: where e^x - 1 is 0 (B_nu infinite) it is 0, and
-        // where e^x overflows (B_nu 0) it is infinite, or NaN for j = 0 -- the
-        // reference's divisions give the same
-        const double alpha = j * (kE2CH / kE2C) * (dexp(x) - 1.) * (0.5 * kC * kC) *
-                             drcp(nu_p * nu_p * nu_p);
-        const double dtau = nu_p * drcp(nu) * alpha * dl;
-        double tau = tau_f64(acc.tau[f], acc.ctau[f]) + dtau;
+        const SampleF64 T = sample_terms_f64<GRAD>(F, S, P.freq[f0 + f], dl);
+        const double tau = tau_f64(acc.tau[f], acc.ctau[f]) + T.dtau;
         set_tau_f64(acc.tau[f], acc.ctau[f], tau);
-        const double gi = nu * drcp(nu_p);
-        const double dI = j * dexp(-tau) * dl * (gi * gi);
-        double dlnj0 = 0., dlnj1 = 0., dlna1 = 0.;
-        if (GRAD && j != 0.) {
-            const double cb = s6 * s6, sx = cb * s6;
-            const double Dx = 2. * (0.5 * sx + c1112 / 6. * s6) * drcp(sx + c1112 * s6) - cb / 3.;
-            const double dlnB = 0.5 * (F.B - 1e-20) * drcp(F.B);
-            const double dlnT = F.dthetae * drcp(F.thetae);
-            const double dlnBnu = x * drcp(-dexpm1(-x));
-            dlnj0 = 1. + dlnB * (1. - Dx);
-            dlnj1 = -2. * Dx * dlnT;
-            dlna1 = dlnj1 - dlnBnu * dlnT;
-        }
-        if (GRAD && dtau != 0.) {
-            acc.dtau[f][0] += dtau * dlnj0;
-            acc.dtau[f][1] += dtau * dlna1;
-        }
-        if (tau < lnth && !(tau < 0.)) {
-            acc.I[f] += (float)dI;
-            if (GRAD && dI != 0.) {
-                acc.dI[f][0] += dI * (dlnj0 - acc.dtau[f][0]);
-                acc.dI[f][1] += dI * (dlnj1 - acc.dtau[f][1]);
-            }
-        }
+        add_sample_f64(acc, f, T, tau, dexp(-tau));
     }
     return true;
 }
