@@ -203,6 +203,41 @@ def test_c2_batched_equals_per_image(tracer):
             assert spec[c, f] == s1[0, 0]
 
 
+@pytest.mark.parametrize("dim3,grad,integrator,approach", [
+    (False, False, rt.INT_RK2, 100), (False, True, rt.INT_RK2, None),
+    (True, True, rt.INT_RK2, 100), (False, False, rt.INT_RK4, None),
+    (True, False, rt.INT_VERLET, 100)])
+def test_batched_frequencies_equal_single_launches(tracer, dump3d_small, dump128, dim3, grad,
+                                                   integrator, approach):
+    """Three frequencies in one launch equal three one-frequency launches,
+    image and tangents, bit for bit: the rays whose FP32 optical depth met
+    the gate where the decision shows are traced again with the FP64
+    transfer inline in a one-frequency launch, and by exact_kernel after a
+    batched one (a lane per (ray, frequency), or a wave per ray with one
+    frequency in the gate window).  256^2 rays (on the 2D dump the FP64
+    oracle's optical depth crosses the gate within 1e-3 of ln 1000 on 33-36
+    rays per frequency), 2D and 3D dumps, the three integrators, the split
+    launch (approach kernel, resumed rays) and the one-kernel launch."""
+    freqs = [1.3e11, 2.3e11, 3.45e11]
+    W = 256
+    tracer.set_params(width=W, height=W, M_unit=M_UNIT, R_high=R_HIGH, integrator=integrator)
+    tracer.load_dump(dump3d_small if dim3 else dump128)
+    try:
+        if approach is not None:
+            tracer.set_launch_config(approach=approach)
+        out = tracer.render([INCL], freqs, grad=grad)
+        for f, nu in enumerate(freqs):
+            one = tracer.render([INCL], [nu], grad=grad)
+            np.testing.assert_array_equal(out[0][0, :, f], one[0][0, :, 0])
+            if grad:
+                np.testing.assert_array_equal(out[1][0, :, f], one[1][0, :, 0])
+        assert out[0].max() > 0
+    finally:
+        tracer.set_launch_config()
+        tracer.set_params(width=W, height=W, M_unit=M_UNIT, R_high=R_HIGH)
+        tracer.load_dump(dump128)
+
+
 def test_graph_replay_matches_eager(tracer):
     """A captured graph of two renders, replayed several times with eager
     work in between; every replay equals the eager result."""
