@@ -312,11 +312,13 @@ __device__ __forceinline__ void trace_rays(const Params &P, const TraceArgs &A)
     // resumed RK2 rays overlap each transfer sample with the next step's
     // first acceleration (ray_step_rk2_overlap), carried in A0
     constexpr bool kOverlap = RESUME && INT == kRK2;
-    // one frequency of the thermal fit: a ray whose FP32 optical depth met
-    // the ln 1000 gate where a different decision would show in the pixel
-    // (needs_exact) is traced again at once by its lane with the FP64
-    // transfer (xm); with several frequencies it is listed for exact_kernel
-    constexpr bool kInline = EMIS == kThSynch && NF == 1;
+    // a ray whose FP32 optical depth met the ln 1000 gate where a different
+    // decision would show in the pixel (needs_exact): with one frequency of
+    // the thermal fit, in the kernels tracing from the camera, its lane
+    // traces it again at once with the FP64 transfer (xm); the resuming
+    // kernels (whose registers that branch would spill: 1024^2 render -6%
+    // without it) and those of several frequencies list it for exact_kernel
+    constexpr bool kInline = EMIS == kThSynch && NF == 1 && !RESUME;
     Ray<NF, GRAD> R;
     double A0[4] = {0., 0., 0., 0.};
     int ray = 0, slot = 0;
@@ -402,22 +404,6 @@ __global__ __launch_bounds__(64, 3) void trace_kernel_3w(const Params P, const T
     trace_rays<NF, GRAD, DIM3, INT, RESUME, EMIS>(P, A);
 }
 
-// The rays a trace kernel listed (needs_exact) traced again from their start
-// (the camera, or the saved ray at the grid's edge) by exact_retrace, one
-// frequency per lane: workgroup row blockIdx.y takes frequency f of every
-// listed ray whose gate mask has bit f (the frequencies that had a sample in
-// the gate window; for the others the FP32 outputs stand).  Where that
-// frequency's own share of the window exceeds kGateFlip it gets the FP64
-// transfer's outputs, written over its FP32 ones -- the same decision and
-// the same arithmetic as a launch of that frequency alone, whose trace
-// kernel traces the ray again itself.  With one frequency in the mask its
-// share was the ray's whole sum, so the decision is known and only the FP64
-// transfer runs.  A lane's second trace ends where its
-// frequency's optical depth has passed the gate, so that it runs from the
-// grid's edge to the crossing, not to the end of the ray at every
-// frequency.  A kernel of its own: its FP64 arithmetic needs more registers
-// than the multi-frequency trace kernels can give.  Consecutive entries go
-// to different waves, so that those few rays run on different SIMDs.
 // Lane k's value of v (k the same on every lane).
 __device__ inline double lane_read(double v, int k)
 {
@@ -512,11 +498,35 @@ __device__ void exact_retrace_wave(const Params &P, const float4 *__restrict__ p
             set_tau_f64(x.tau[0], x.ctau[0], tk);
             add_sample_f64(x, 0, t, tk, lane_read(metau, k));
         }
-        if (last < 64)
+        // the ray's counters as the one-lane re-trace leaves them
+        if (last < 64) {
+            R.steps -= nb - 1 - last;
+            R.samples += __popcll(okm & ((2ull << last) - 1ull));
             done = true;
+        } else {
+            R.samples += __popcll(okm);
+        }
     }
 }
 
+// The rays a trace kernel listed (needs_exact) traced again from their start
+// (the camera, or the saved ray at the grid's edge) with the FP64 transfer.
+// Rows blockIdx.y < n_freq, one frequency per lane: row f takes frequency f
+// of every listed ray whose gate mask has bit f (the frequencies that had a
+// sample in the gate window; for the others the FP32 outputs stand) and
+// several bits; where that frequency's own share of the window exceeds
+// kGateFlip it gets the FP64 transfer's outputs, written over its FP32 ones
+// -- the same decision and the same arithmetic as a launch of that
+// frequency alone, which traces the ray again in its trace kernel from the
+// camera, or here.  A ray with one bit was listed for that frequency's own
+// share, so the decision is known and only the FP64 transfer runs, by a
+// whole wave in the rows past n_freq (exact_retrace_wave).  A second trace
+// ends where its frequency's optical depth has passed the gate, so that it
+// runs from the grid's edge to the crossing, not to the end of the ray at
+// every frequency.  A kernel of its own: its FP64 arithmetic needs more
+// registers than the resuming and multi-frequency trace kernels can give.
+// Consecutive entries go to different waves, so that those few rays run on
+// different SIMDs.
 template <bool GRAD, bool DIM3, int INT, bool RESUME>
 __global__ __launch_bounds__(64) void exact_kernel(const Params P, const TraceArgs A)
 {
@@ -549,6 +559,14 @@ __global__ __launch_bounds__(64) void exact_kernel(const Params P, const TraceAr
                 if (GRAD) {
                     A.dimage[o * 2 + 0] = x.dI[0][0];
                     A.dimage[o * 2 + 1] = x.dI[0][1];
+                }
+                // one frequency: the counters of the re-trace, as a trace
+                // kernel that traces its rays again itself writes them
+                if (A.counters && P.nf == 1) {
+                    rt_ray_counters c;
+                    c.steps = R.steps;
+                    c.samples = R.samples;
+                    A.counters[ray] = c;
                 }
             }
         }
@@ -1049,14 +1067,14 @@ static hipError_t dispatch_nf(rt_ctx *c, const Params &P, const TraceArgs &A,
     return launch_trace<16, GRAD, DIM3, INT, RESUME>(c, P, A, st);
 }
 
-// The FP64 re-trace after a multi-frequency trace kernel (thermal fit only;
-// with one frequency the trace kernel traces its rays again itself): n_freq
-// rows of up to 4 workgroups per CU, each lane taking list entries in a
-// row-sized stride.
+// The FP64 re-trace after a resuming or multi-frequency trace kernel
+// (thermal fit only; a one-frequency kernel tracing from the camera traces
+// its rays again itself): n_freq rows of up to 4 workgroups per CU, each lane
+// taking list entries in a row-sized stride.
 template <bool GRAD, bool DIM3, int INT, bool RESUME>
 static hipError_t dispatch_exact(rt_ctx *c, const Params &P, const TraceArgs &A)
 {
-    if (P.emission != kThSynch || P.nf == 1)
+    if (P.emission != kThSynch || (P.nf == 1 && !RESUME))
         return hipSuccess;
     const long long gx = std::max(1LL, std::min<long long>(4LL * c->num_cu,
                                                            ((long long)A.total + 63) / 64));

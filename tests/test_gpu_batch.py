@@ -99,6 +99,35 @@ def test_split_equals_fused_with_gradients_and_counters(tracer):
             tracer.free(p)
 
 
+@pytest.mark.parametrize("grad", [False, True])
+def test_split_equals_fused_where_rays_are_traced_again(tracer, grad):
+    """One frequency at 256^2, where a few tens of rays meet the gate window:
+    the split launch lists them for exact_kernel, the one-kernel launch
+    traces them again in its own lanes; image, tangents and per-ray
+    counters agree bit for bit."""
+    W = 256
+    _params(tracer, W)
+    n = W * W
+    d, dc = tracer.alloc(n * 8), tracer.alloc(n * 8)
+    dd = tracer.alloc(n * 16) if grad else None
+    try:
+        out = []
+        for approach in (rt.APPROACH_NONE, 100):
+            tracer.set_launch_config(approach=approach)
+            tracer.render_device([INCL], [FREQ], d, dd, dc)
+            tracer.synchronize()
+            out.append([tracer.d2h(d, (n,), np.float64), tracer.d2h(dc, (n, 2), np.int32)] +
+                       ([tracer.d2h(dd, (n, 2), np.float64)] if grad else []))
+        for a, b in zip(out[0], out[1]):
+            assert np.array_equal(a, b)
+        assert out[0][0].max() > 0
+    finally:
+        tracer.set_launch_config()
+        for p in (d, dc, dd):
+            if p is not None:
+                tracer.free(p)
+
+
 def test_split_capture_chunks_and_pins_its_ray_buffer(dump128):
     """The buffer of saved rays is sized by eager renders.  A capture larger
     than the buffer runs in chunks that fit it; a later eager render that
